@@ -1,0 +1,629 @@
+"""CPU oracle for the joint motion/image reconstruction hot path.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product package imports, links or
+executes this module.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` use it, and only as the checker (or, for
+``cpu_baseline``, as the timed CPU restatement), never as the thing measured on
+the GPU.
+
+It is a plain NumPy/SciPy restatement of the reference's algorithm, following
+the reference op by op (dense masks, 3-D FFTs, dense inner products), in one of
+two precisions:
+
+* ``prec="f64"``: complex128 everywhere (the FP64 CPU oracle);
+* ``prec="c64"``: the reference's own dtypes (complex64 data, complex64 shear
+  phases in ``apply``, complex128 phases rounded at the multiply elsewhere,
+  FP32 CG dot products, FP64 6x6 LM quantities).
+
+Each function cites the reference file:line it restates.  The reference itself
+cannot run here (CuPy/SigPy are absent and its CG/LM path is CUDA-only); every
+function is pinned by the known-answer tests, invariants and finite-difference
+checks in ``tests/test_oracle.py``.  Parity status per function:
+
+==============================  ===============================================
+function                        pinned by
+==============================  ===============================================
+compute_transform_grids_voxel   closed form (fftfreq) test
+rigid_apply / rigid_adjoint     integer roll KAT, 90-degree permutation KAT,
+                                unitarity, adjoint identity, Gaussian rotation
+                                vs analytic, rotation-order test
+rigid_deriv                     central finite differences of the reverse-order
+                                composition Tr*Rx*Ry*Rz, out-of-range p_idx
+normal operator / CG            explicit matrix from basis vectors + textbook
+                                CG; closed form sum_s E_s^H E_s = I
+estimate_transform              6x6 solve vs numpy.linalg.solve on explicit
+                                Jacobian; translation-only convergence KAT;
+                                Jacobian vs finite differences of the residual
+make_disorder_order             bit-exact stdlib ``random`` (the reference uses
+                                ``random.Random``); tile/union invariants
+pyramid resampling              SigPy centred fft/resize restated; crop KATs
+==============================  ===============================================
+"""
+from __future__ import annotations
+
+import math
+import random
+from dataclasses import dataclass
+from typing import List, Tuple
+
+import numpy as np
+
+try:  # scipy.fft is faster and multithreaded; numpy.fft is the fallback
+    import scipy.fft as _sfft
+
+    def _fft(a, axis, workers):
+        return _sfft.fft(a, axis=axis, norm="ortho", workers=workers)
+
+    def _ifft(a, axis, workers):
+        return _sfft.ifft(a, axis=axis, norm="ortho", workers=workers)
+
+    def _fftn(a, axes, workers):
+        return _sfft.fftn(a, axes=axes, norm="ortho", workers=workers)
+
+    def _ifftn(a, axes, workers):
+        return _sfft.ifftn(a, axes=axes, norm="ortho", workers=workers)
+
+except ImportError:  # pragma: no cover
+    def _fft(a, axis, workers):
+        return np.fft.fft(a, axis=axis, norm="ortho")
+
+    def _ifft(a, axis, workers):
+        return np.fft.ifft(a, axis=axis, norm="ortho")
+
+    def _fftn(a, axes, workers):
+        return np.fft.fftn(a, axes=axes, norm="ortho")
+
+    def _ifftn(a, axes, workers):
+        return np.fft.ifftn(a, axes=axes, norm="ortho")
+
+
+WORKERS = 1
+
+
+def set_workers(n: int) -> None:
+    """Number of threads scipy.fft may use (the CPU baseline sets this)."""
+    global WORKERS
+    WORKERS = max(1, int(n))
+
+
+def _cdt(prec):
+    return np.complex128 if prec == "f64" else np.complex64
+
+
+def _rdt(prec):
+    return np.float64 if prec == "f64" else np.float32
+
+
+# ---------------------------------------------------------------------------
+# a1: grids  (utils.py:107-137)
+# ---------------------------------------------------------------------------
+def compute_transform_grids_voxel(shape, voxel_size, downsample_res=None):
+    """utils.py:107-137.  r in full-resolution voxel units (fftshifted
+    linspace over the full extent, sampled at the level size); k the
+    fftshifted linspace(-pi, pi) at the level size."""
+    nx, ny, nz = shape
+    vx, vy, vz = voxel_size
+    if downsample_res is None:
+        downsample_res = shape
+    dsx, dsy, dsz = downsample_res
+    rx = np.fft.fftshift((np.linspace(-nx // 2, nx // 2, dsx, endpoint=False) * vx).reshape(-1, 1, 1))
+    ry = np.fft.fftshift((np.linspace(-ny // 2, ny // 2, dsy, endpoint=False) * vy).reshape(1, -1, 1))
+    rz = np.fft.fftshift((np.linspace(-nz // 2, nz // 2, dsz, endpoint=False) * vz).reshape(1, 1, -1))
+    kx = np.fft.fftshift(np.linspace(-np.pi, np.pi, dsx, endpoint=False).reshape((-1, 1, 1)))
+    ky = np.fft.fftshift(np.linspace(-np.pi, np.pi, dsy, endpoint=False).reshape((1, -1, 1)))
+    kz = np.fft.fftshift(np.linspace(-np.pi, np.pi, dsz, endpoint=False).reshape((1, 1, -1)))
+    kgrid = {"x": kx, "y": ky, "z": kz}
+    rkgrid = {
+        "tan": {"x": rz * ky, "y": rx * kz, "z": ry * kx},
+        "sin": {"x": ry * kz, "y": rz * kx, "z": rx * ky},
+    }
+    return kgrid, rkgrid
+
+
+# ---------------------------------------------------------------------------
+# a2-a4: rigid transform and adjoint  (transform.py:5-14, 243-377)
+# ---------------------------------------------------------------------------
+def shear_factor(rk, theta, mode, inverse=False):
+    """transform.py:5-14."""
+    if mode == "tan":
+        f = np.exp(1j * np.tan(theta / 2) * rk)
+    elif mode == "sin":
+        f = np.exp(-1j * np.sin(theta) * rk)
+    else:
+        raise ValueError("Mode must be either 'tan' or 'sin'!")
+    return f.conj() if inverse else f
+
+
+def _steps(params, rkgrid):
+    q1, q2, q3, t1, t2, t3 = [float(v) for v in params]
+    return (q1, q2, q3), [
+        (t1, rkgrid["tan"]["x"], rkgrid["sin"]["x"], 1, 2),
+        (t2, rkgrid["tan"]["y"], rkgrid["sin"]["y"], 2, 0),
+        (t3, rkgrid["tan"]["z"], rkgrid["sin"]["z"], 0, 1),
+    ]
+
+
+def _shear(arr, axis, phase, w):
+    return _ifft(_fft(arr, axis, w) * phase, axis, w)
+
+
+def translation_factor(kgrid, q1, q2, q3, inverse=False):
+    """transform.py:257-261 (complex128, like the reference)."""
+    f = np.exp(-1j * (q1 * kgrid["x"] + q2 * kgrid["y"] + q3 * kgrid["z"]))
+    return f.conj() if inverse else f
+
+
+def rigid_apply(x, params, kgrid, rkgrid, inverse=False, prec="f64"):
+    """RigidTransformCudaOptimzied.apply, transform.py:263-318.
+
+    T x = Tr(q) Rz(t3) Ry(t2) Rx(t1) x with each R three 1-D Fourier shears
+    (tan, sin, tan) and Tr a 3-D Fourier shift.  ``inverse`` only conjugates
+    the translation (transform.py:308, quirk)."""
+    cdt = _cdt(prec)
+    w = WORKERS
+    arr = np.array(x, dtype=cdt, copy=True)
+    (q1, q2, q3), steps = _steps(params, rkgrid)
+    for theta, rkt, rks, at, as_ in steps:
+        if prec == "f64":
+            vtan = np.exp(1j * np.tan(theta / 2) * rkt)
+            vsin = np.exp(-1j * np.sin(theta) * rks)
+        else:  # transform.py:275-276: exp evaluated in complex64
+            vtan = np.exp((1j * np.tan(theta / 2) * rkt).astype(np.complex64))
+            vsin = np.exp((-1j * np.sin(theta) * rks).astype(np.complex64))
+        arr = _shear(arr, at, vtan, w).astype(cdt, copy=False)
+        arr = _shear(arr, as_, vsin, w).astype(cdt, copy=False)
+        arr = _shear(arr, at, vtan, w).astype(cdt, copy=False)
+    U = translation_factor(kgrid, q1, q2, q3, inverse)
+    arr = _ifftn(_fftn(arr, (0, 1, 2), w) * U, (0, 1, 2), w).astype(cdt, copy=False)
+    return arr
+
+
+def rigid_adjoint(x, params, kgrid, rkgrid, prec="f64"):
+    """RigidTransformCudaOptimzied.adjoint, transform.py:320-377: conjugate
+    translation first, then rotations z, y, x with conjugated shears."""
+    cdt = _cdt(prec)
+    w = WORKERS
+    arr = np.array(x, dtype=cdt, copy=True)
+    (q1, q2, q3), steps = _steps(params, rkgrid)
+    U = translation_factor(kgrid, q1, q2, q3, True)
+    arr = _ifftn(_fftn(arr, (0, 1, 2), w) * U, (0, 1, 2), w).astype(cdt, copy=False)
+    for theta, rkt, rks, at, as_ in steps[::-1]:
+        vtan = shear_factor(rkt, theta, "tan", True)
+        vsin = shear_factor(rks, theta, "sin", True)
+        arr = _shear(arr, at, vtan, w).astype(cdt, copy=False)
+        arr = _shear(arr, as_, vsin, w).astype(cdt, copy=False)
+        arr = _shear(arr, at, vtan, w).astype(cdt, copy=False)
+    return arr
+
+
+# ---------------------------------------------------------------------------
+# a5: derivative operator  (transform.py:379-504)
+# ---------------------------------------------------------------------------
+def rigid_deriv(x, params, kgrid, rkgrid, p_idx, prec="f64"):
+    """RigidTransformDerivativeCuda.apply, transform.py:476-504.
+
+    d/dp of T_d = Tr Rx Ry Rz (rotations applied z, y, x: the REVERSE of
+    ``rigid_apply``; quirk Q1).  p<3: translation derivative
+    F^H(-i k_p U)F after all rotations (transform.py:422-431); p=3,4,5: the
+    three-term product rule over the shears of that rotation
+    (transform.py:433-474).  p outside [0,5] returns a copy of the input
+    (no else branch, transform.py:476-504)."""
+    cdt = _cdt(prec)
+    w = WORKERS
+    q1, q2, q3, t1, t2, t3 = [float(v) for v in params]
+    out = np.array(x, dtype=cdt, copy=True)
+
+    def phases(theta, rkt, rks):
+        if prec == "f64":
+            vtan = np.exp(1j * np.tan(theta / 2) * rkt)
+            vsin = np.exp(-1j * np.sin(theta) * rks)
+        else:
+            vtan = np.exp((1j * np.tan(theta / 2) * rkt).astype(np.complex64))
+            vsin = np.exp((-1j * np.sin(theta) * rks).astype(np.complex64))
+        return vtan, vsin
+
+    def rot(a, theta, rkt, rks, at, as_):  # transform.py:392-409
+        vtan, vsin = phases(theta, rkt, rks)
+        a = _shear(a, at, vtan, w).astype(cdt, copy=False)
+        a = _shear(a, as_, vsin, w).astype(cdt, copy=False)
+        return _shear(a, at, vtan, w).astype(cdt, copy=False)
+
+    def rot_d(a, theta, rkt, rks, at, as_):  # transform.py:433-474
+        vtan, vsin = phases(theta, rkt, rks)
+        vtan_d = (1j * ((1 + np.tan(theta / 2) ** 2) / 2) * rkt * vtan).astype(cdt)
+        vsin_d = (-1j * np.cos(theta) * rks * vsin).astype(cdt)
+        a = _fft(a, at, w).astype(cdt, copy=False)
+        s2 = a.copy()
+        s3 = a.copy()
+        o = _ifft(a * vtan_d, at, w)
+        o = _ifft(_fft(o, as_, w) * vsin, as_, w)
+        o = _fft(o, at, w) * vtan
+        s2 = _ifft(s2 * vtan, at, w)
+        s2 = _ifft(_fft(s2, as_, w) * vsin_d, as_, w)
+        s2 = _fft(s2, at, w) * vtan
+        s3 = _ifft(s3 * vtan, at, w)
+        s3 = _ifft(_fft(s3, as_, w) * vsin, as_, w)
+        s3 = _fft(s3, at, w) * vtan_d
+        return _ifft((o + s2 + s3).astype(cdt, copy=False), at, w).astype(cdt, copy=False)
+
+    def trans(a, deriv_axis=None):  # transform.py:411-431
+        U = translation_factor(kgrid, q1, q2, q3)
+        if deriv_axis is not None:
+            U = -1j * kgrid["xyz"[deriv_axis]] * U
+        return _ifftn(_fftn(a, (0, 1, 2), w) * U, (0, 1, 2), w).astype(cdt, copy=False)
+
+    Z = (t3, rkgrid["tan"]["z"], rkgrid["sin"]["z"], 0, 1)
+    Y = (t2, rkgrid["tan"]["y"], rkgrid["sin"]["y"], 2, 0)
+    X = (t1, rkgrid["tan"]["x"], rkgrid["sin"]["x"], 1, 2)
+    if p_idx < 3:
+        out = rot(out, *Z)
+        out = rot(out, *Y)
+        out = rot(out, *X)
+        out = trans(out, p_idx)
+    elif p_idx == 3:
+        out = rot(out, *Z)
+        out = rot(out, *Y)
+        out = rot_d(out, *X)
+        out = trans(out)
+    elif p_idx == 4:
+        out = rot(out, *Z)
+        out = rot_d(out, *Y)
+        out = rot(out, *X)
+        out = trans(out)
+    elif p_idx == 5:
+        out = rot_d(out, *Z)
+        out = rot(out, *Y)
+        out = rot(out, *X)
+        out = trans(out)
+    return out
+
+
+def rigid_apply_reverse(x, params, kgrid, rkgrid, prec="f64"):
+    """T_d x = Tr Rx Ry Rz x: the composition whose Jacobian ``rigid_deriv``
+    returns (rotations z, y, x as in transform.py:481-483, then
+    transform.py:411-420).  Used by the finite-difference pin."""
+    cdt = _cdt(prec)
+    w = WORKERS
+    q1, q2, q3, t1, t2, t3 = [float(v) for v in params]
+    a = np.array(x, dtype=cdt, copy=True)
+    for theta, rkt, rks, at, as_ in [
+        (t3, rkgrid["tan"]["z"], rkgrid["sin"]["z"], 0, 1),
+        (t2, rkgrid["tan"]["y"], rkgrid["sin"]["y"], 2, 0),
+        (t1, rkgrid["tan"]["x"], rkgrid["sin"]["x"], 1, 2),
+    ]:
+        vtan = np.exp(1j * np.tan(theta / 2) * rkt)
+        vsin = np.exp(-1j * np.sin(theta) * rks)
+        a = _shear(a, at, vtan, w)
+        a = _shear(a, as_, vsin, w)
+        a = _shear(a, at, vtan, w)
+    U = translation_factor(kgrid, q1, q2, q3)
+    return _ifftn(_fftn(a, (0, 1, 2), w) * U, (0, 1, 2), w).astype(cdt, copy=False)
+
+
+# ---------------------------------------------------------------------------
+# encoding helpers (inlined in the reference: image_estimation.py:93-129,
+# transform_estimation.py:41-55, joint_recon.py:111-113)
+# ---------------------------------------------------------------------------
+def _vdot_real(a, b, prec):
+    if prec == "f64":
+        return float(np.vdot(a.astype(np.complex128, copy=False), b.astype(np.complex128, copy=False)).real)
+    return float(np.float32(np.vdot(a, b).real))
+
+
+def encode_forward(x, mps, mask_s, params, kgrid, rkgrid, prec="f64"):
+    """mask_s * fftn(mps * T x)  (transform_estimation.py:207-210 without the
+    subtraction; joint_recon.py:428-429)."""
+    cdt = _cdt(prec)
+    Tx = rigid_apply(x, params, kgrid, rkgrid, prec=prec)
+    return (_fftn((mps * Tx).astype(cdt, copy=False), (-3, -2, -1), WORKERS) * mask_s).astype(cdt, copy=False)
+
+
+def encode_adjoint(y, mps, mask_s, params, kgrid, rkgrid, prec="f64"):
+    """T^H sum_c conj(mps) ifftn(mask_s * y)  (image_estimation.py:94-102)."""
+    cdt = _cdt(prec)
+    t = _ifftn((mask_s * y).astype(cdt, copy=False), (-3, -2, -1), WORKERS)
+    return rigid_adjoint(np.sum(np.conj(mps) * t, axis=0), params, kgrid, rkgrid, prec=prec)
+
+
+def cg_rhs(ksp, mps, shot_mask, transforms, kgrid, rkgrid, M, prec="f64"):
+    """image_estimation.py:92-103."""
+    cdt = _cdt(prec)
+    b = np.zeros(ksp.shape[1:], dtype=cdt)
+    for i in range(transforms.shape[0]):
+        temp = _ifftn((shot_mask[i] * ksp).astype(cdt, copy=False), (-3, -2, -1), WORKERS)
+        b += rigid_adjoint(np.sum(np.conj(mps) * temp, axis=0), transforms[i], kgrid, rkgrid, prec=prec)
+        b *= M
+    return b
+
+
+def normal_op(x, mps, shot_mask, transforms, kgrid, rkgrid, M, prec="f64"):
+    """A_func, image_estimation.py:106-129: M * sum_s T_s^H sum_c conj(S_c)
+    F^H M_s F S_c T_s x."""
+    cdt = _cdt(prec)
+    out = np.zeros(x.shape, dtype=cdt)
+    for i in range(transforms.shape[0]):
+        Ex = rigid_apply(x, transforms[i], kgrid, rkgrid, prec=prec)
+        Ex = (mps * Ex).astype(cdt, copy=False)
+        Ex = _fftn(Ex, (-3, -2, -1), WORKERS) * shot_mask[i]
+        Ex = _ifftn(Ex.astype(cdt, copy=False), (-3, -2, -1), WORKERS)
+        out += rigid_adjoint(np.sum(np.conj(mps) * Ex, axis=0), transforms[i], kgrid, rkgrid, prec=prec)
+    out *= M
+    return out
+
+
+def estimate_image_cg(ksp, mps, shot_mask, transforms, kgrid, rkgrid, P, M, x0=None, max_iter=100,
+                      tol=1e-6, prec="f64", history=None):
+    """image_estimation.py:10-165 (PCG).  ``history``, if a list, receives
+    (alpha, beta, rsnew) per iteration."""
+    cdt = _cdt(prec)
+    x = np.zeros(ksp.shape[1:], dtype=cdt) if x0 is None else np.array(x0, dtype=cdt, copy=True)
+    b = cg_rhs(ksp, mps, shot_mask, transforms, kgrid, rkgrid, M, prec)
+    Ap = normal_op(x, mps, shot_mask, transforms, kgrid, rkgrid, M, prec)
+    r = b - Ap
+    z = (P * r).astype(cdt, copy=False)
+    p = z.copy()
+    rsold = _vdot_real(r, z, prec)
+    for _ in range(max_iter):
+        Ap = normal_op(p, mps, shot_mask, transforms, kgrid, rkgrid, M, prec)
+        pAp = _vdot_real(p, Ap, prec)
+        alpha = rsold / pAp
+        x = (x + alpha * p).astype(cdt, copy=False)
+        r = (r - alpha * Ap).astype(cdt, copy=False)
+        z = (P * r).astype(cdt, copy=False)
+        rsnew = _vdot_real(r, z, prec)
+        if history is not None:
+            history.append((alpha, rsnew / rsold, rsnew))
+        if math.sqrt(abs(rsnew)) < tol:
+            break
+        beta = rsnew / rsold
+        p = (z + beta * p).astype(cdt, copy=False)
+        rsold = rsnew
+    return x
+
+
+# ---------------------------------------------------------------------------
+# a9-a10: LM motion update  (transform_estimation.py:7-152, use_ga=False)
+# ---------------------------------------------------------------------------
+def shot_residual(ksp, mps, mask_s, image, params, kgrid, rkgrid, prec="f64"):
+    """transform_estimation.py:206-210."""
+    cdt = _cdt(prec)
+    Tx = rigid_apply(image, params, kgrid, rkgrid, prec=prec)
+    r = _fftn((mps * Tx).astype(cdt, copy=False), (-3, -2, -1), WORKERS)
+    return ((r - ksp) * mask_s).astype(cdt, copy=False)
+
+
+def shot_jacobian(mps, mask_s, image, params, kgrid, rkgrid, prec="f64"):
+    """transform_estimation.py:216-220: six columns mask*fftn(mps*dT_p x)."""
+    cdt = _cdt(prec)
+    cols = []
+    for p_idx in range(6):
+        d = rigid_deriv(image, params, kgrid, rkgrid, p_idx, prec=prec)
+        cols.append((_fftn((mps * d).astype(cdt, copy=False), (-3, -2, -1), WORKERS) * mask_s).astype(cdt, copy=False))
+    return cols
+
+
+def lm_normal_equations(residual, cols, damp_s, prec="f64"):
+    """transform_estimation.py:212-233: e0, gradient, damped Gram matrix."""
+    e0 = _vdot_real(residual, residual, prec)
+    g = np.array([_vdot_real(residual, c, prec) for c in cols], dtype=np.float64)
+    H = np.zeros((6, 6), dtype=np.float64)
+    for i in range(6):
+        for j in range(i, 6):
+            v = _vdot_real(cols[i], cols[j], prec)
+            if i == j:
+                H[i, j] = v + damp_s
+            else:
+                H[i, j] = v
+                H[j, i] = v
+    return e0, g, H
+
+
+def estimate_transform(ksp, mps, shot_mask, image, kgrid, rkgrid, damp, convergence_flags, transforms,
+                       tol=1e-6, prec="f64", trace=None):
+    """transform_estimation.py:7-152 with use_ga=False (the default).
+
+    Mutates ``transforms``, ``damp`` and ``convergence_flags`` in place and
+    returns T(mean) image.  ``trace``, if a list, receives per-shot
+    (e0, e1, accepted, delta)."""
+    nshots = shot_mask.shape[0]
+    for s in range(nshots):
+        res = shot_residual(ksp, mps, shot_mask[s], image, transforms[s], kgrid, rkgrid, prec)
+        cols = shot_jacobian(mps, shot_mask[s], image, transforms[s], kgrid, rkgrid, prec)
+        e0, g, H = lm_normal_equations(res, cols, damp[s], prec)
+        delta = np.linalg.solve(H, g)  # transform_estimation.py:236
+        nxt = transforms[s] - delta  # transform_estimation.py:290
+        res1 = shot_residual(ksp, mps, shot_mask[s], image, nxt, kgrid, rkgrid, prec)
+        e1 = _vdot_real(res1, res1, prec)
+        acc = e1 < e0
+        if acc:  # transform_estimation.py:303-309
+            if np.all(np.abs(transforms[s] - nxt) < tol):
+                convergence_flags[s] = True
+            transforms[s] = nxt.copy()
+            damp[s] = max(damp[s] / 5, 1e-4)
+        else:
+            damp[s] = min(damp[s] * 1.5, 1e16)
+        if trace is not None:
+            trace.append((e0, e1, bool(acc), delta.copy()))
+    mean_t = np.mean(transforms, axis=0)  # transform_estimation.py:312-315
+    out = rigid_apply(image, mean_t, kgrid, rkgrid, prec=prec)
+    transforms -= mean_t
+    return out
+
+
+# ---------------------------------------------------------------------------
+# a11-a12: JointRecon  (joint_recon.py:38-115)
+# ---------------------------------------------------------------------------
+class JointRecon:
+    """joint_recon.py:10-115 on the CPU.  Inputs are centred (the class
+    uncentres them, joint_recon.py:63-65)."""
+
+    def __init__(self, ksp, mps, shot_mask, kgrid, rkgrid, t0=None, max_cg_iter=3, max_nm_iter=1,
+                 max_joint_iter=100, prec="f64", compute_objective=True):
+        cdt = _cdt(prec)
+        self.prec = prec
+        self.kgrid, self.rkgrid = kgrid, rkgrid
+        self.max_cg_iter, self.max_nm_iter, self.max_joint_iter = max_cg_iter, max_nm_iter, max_joint_iter
+        self.compute_objective = compute_objective
+        self.num_shots = len(shot_mask)
+        self.img_shape = ksp.shape[1:]
+        self.ksp = np.fft.ifftshift(np.asarray(ksp), axes=(-3, -2, -1)).astype(cdt)
+        self.mps = np.fft.ifftshift(np.asarray(mps), axes=(-3, -2, -1)).astype(cdt)
+        self.shot_mask = np.fft.ifftshift(np.asarray(shot_mask), axes=(-3, -2, -1))
+        self.max_norm = float(np.max(np.abs(_ifftn(self.ksp, (-3, -2, -1), WORKERS))))
+        self.ksp /= self.max_norm
+        self.x = np.zeros(self.img_shape, dtype=cdt)
+        self.t = np.zeros((self.num_shots, 6), dtype=np.float64) if t0 is None else np.array(t0, dtype=np.float64)
+        self.damp = np.ones(self.num_shots)
+        self.convergence_flags = np.zeros(self.num_shots, dtype=bool)
+        ssq = np.sum(np.abs(self.mps) ** 2, axis=0)
+        self.P = (1 / (ssq + 1e-3)).astype(_rdt(prec))
+        self.M = ssq > 0.1
+
+    def step(self):
+        """One outer iteration of joint_recon.py:92-102."""
+        self.x = estimate_image_cg(self.ksp, self.mps, self.shot_mask, self.t, self.kgrid, self.rkgrid,
+                                   self.P, self.M, x0=self.x, max_iter=self.max_cg_iter, prec=self.prec)
+        for _ in range(self.max_nm_iter):
+            self.x = estimate_transform(self.ksp, self.mps, self.shot_mask, self.x, self.kgrid, self.rkgrid,
+                                        self.damp, self.convergence_flags, self.t, prec=self.prec)
+            self.x *= self.M
+
+    def objective(self):
+        """joint_recon.py:108-115."""
+        total = 0.0
+        for s in range(self.num_shots):
+            r = shot_residual(self.ksp, self.mps, self.shot_mask[s], self.x, self.t[s], self.kgrid,
+                              self.rkgrid, self.prec)
+            total += _vdot_real(r, r, self.prec)
+        return total
+
+    def run(self):
+        self.transform_history, self.objective_history = [], []
+        for _ in range(self.max_joint_iter):
+            self.step()
+            self.transform_history.append(self.t.copy())
+            if self.compute_objective:
+                self.objective_history.append(self.objective())
+        self.transform_history = np.stack(self.transform_history)
+        self.objective_history = np.array(self.objective_history)
+        return np.fft.fftshift(self.x * self.max_norm), self.t
+
+
+# ---------------------------------------------------------------------------
+# a13: pyramid  (pyramid.py:24-118) with SigPy's centred fft / resize
+# ---------------------------------------------------------------------------
+def sp_resize(a, oshape):
+    """SigPy ``resize`` semantics: centred crop/zero-pad per axis
+    (ishift = max(i//2 - o//2, 0), oshift = max(o//2 - i//2, 0))."""
+    a = np.asarray(a)
+    out = np.zeros(oshape, dtype=a.dtype)
+    src, dst = [], []
+    for i, o in zip(a.shape, oshape):
+        ishift = max(i // 2 - o // 2, 0)
+        oshift = max(o // 2 - i // 2, 0)
+        n = min(i - ishift, o - oshift)
+        src.append(slice(ishift, ishift + n))
+        dst.append(slice(oshift, oshift + n))
+    out[tuple(dst)] = a[tuple(src)]
+    return out
+
+
+def sp_fft(a, axes):
+    """SigPy centred orthonormal FFT over ``axes``."""
+    return np.fft.fftshift(_fftn(np.fft.ifftshift(a, axes=axes), axes, WORKERS), axes=axes)
+
+
+def sp_ifft(a, axes):
+    """SigPy centred orthonormal inverse FFT over ``axes``."""
+    return np.fft.fftshift(_ifftn(np.fft.ifftshift(a, axes=axes), axes, WORKERS), axes=axes)
+
+
+def pyramid_levels(ksp, mps, A, nshots, n_spatial_levels=3, sampling="disorder"):
+    """pyramid.py:55-72: per coarse level, (down_factor, ksp_res, mps_res,
+    A_res, ds_shape)."""
+    full = ksp.shape[1:]
+    ncoils = ksp.shape[0]
+    out = []
+    for sl in range(n_spatial_levels - 1):
+        f = 2 ** (n_spatial_levels - sl - 1)
+        if sampling == "disorder":
+            ds = (int(full[0] // f), int(full[1] // f), full[2])
+        else:
+            ds = (int(full[0] // f), full[1], full[2])
+        mps_res = sp_ifft(sp_resize(sp_fft(mps, (-3, -2, -1)), (ncoils, *ds)), (-3, -2, -1))
+        ksp_res = sp_resize(ksp, (ncoils, *ds))
+        A_res = sp_resize(A, (nshots, *ds[:2], 1))
+        out.append((f, ksp_res, mps_res, A_res, ds))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# a14-a15: DISORDER sampling and the corrupted forward model
+# (simulation.py:65-85, 192-245, 383-410)
+# ---------------------------------------------------------------------------
+@dataclass
+class DisorderConfig:
+    """simulation.py:192-199."""
+    max_line_number: int
+    max_partition_number: int
+    pat_lines_to_measure: int
+    block_lin: int
+    block_par: int
+    seed: int = 123456
+
+
+def make_disorder_order(cfg: DisorderConfig) -> Tuple[List[Tuple[int, int]], Tuple[int, int]]:
+    """simulation.py:201-245: tiles of (block_lin x block_par), one stdlib
+    ``random.Random(seed)`` shuffle per tile in (linG, parG) order, the
+    first ``indices_per_group`` draws interleaved across tiles."""
+    L = math.ceil((cfg.max_line_number + 1) / cfg.block_lin) * cfg.block_lin
+    P = math.ceil((cfg.max_partition_number + 1) / cfg.block_par) * cfg.block_par
+    max_line_number = L - 1
+    max_partition_number = P - 1
+    n_reorder = cfg.pat_lines_to_measure * (max_partition_number + 1)
+    per_group_max = cfg.block_lin * cfg.block_par
+    n_lin_g = L // cfg.block_lin
+    n_par_g = P // cfg.block_par
+    n_groups = n_lin_g * n_par_g
+    per_group = n_reorder // n_groups
+    rng = random.Random(cfg.seed)
+    base = list(range(per_group_max))
+    out = [None] * (per_group * n_groups)
+    for linG in range(n_lin_g):
+        for parG in range(n_par_g):
+            idx = base[:]
+            rng.shuffle(idx)
+            for j in range(per_group):
+                local = idx[j]
+                ll = local % cfg.block_lin
+                lp = local // cfg.block_lin
+                line = ll + (linG % cfg.block_par) * (max_line_number + 1) // cfg.block_par + (linG // cfg.block_par) * cfg.block_lin
+                part = lp + parG * cfg.block_par
+                out[j * n_groups + linG * n_par_g + parG] = (line, part)
+    return out, (L, P)
+
+
+def disorder_masks(order, nshots, K):
+    """simulation.py:396-402: contiguous chunks of ``len(order)//nshots``
+    samples per shot; mask[s, partition, line] = True; trailing samples
+    dropped.  Returns (nshots, K0, K1, 1) bool (simulation.py:437)."""
+    order = np.asarray(order, dtype=int)
+    A = np.zeros((nshots, *K), dtype=bool)
+    sps = order.shape[0] // nshots
+    for s in range(nshots):
+        A[s, order[s * sps:(s + 1) * sps, 1], order[s * sps:(s + 1) * sps, 0]] = True
+    return A[..., None]
+
+
+def generate_corrupted_kspace(gt_img, mps, A, transforms, prec="f64"):
+    """simulation.py:65-85: centred in, centred out."""
+    cdt = _cdt(prec)
+    gt = np.fft.ifftshift(gt_img)
+    m = np.fft.ifftshift(mps, axes=(-3, -2, -1))
+    a = np.fft.ifftshift(A, axes=(-3, -2, -1))
+    kgrid, rkgrid = compute_transform_grids_voxel(gt.shape, [1, 1, 1])
+    ksp = np.zeros(m.shape, dtype=cdt)
+    for s in range(transforms.shape[0]):
+        t = rigid_apply(gt, transforms[s], kgrid, rkgrid, prec=prec)
+        ksp += (_fftn((t * m).astype(cdt, copy=False), (-3, -2, -1), WORKERS) * a[s]).astype(cdt, copy=False)
+    return np.fft.fftshift(ksp, axes=(-3, -2, -1))
