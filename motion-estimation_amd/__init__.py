@@ -1,0 +1,14 @@
+"""MI355X-native joint motion/image reconstruction (gfx950 HIP kernels).
+
+Python mirror of the reference's hot-path API, over libmeamd.so (C ABI in
+include/motion_estimation_amd.h):
+
+* transform: RigidTransformCudaOptimzied, RigidTransformDerivativeCuda
+* utils: compute_transform_grids_voxel
+
+The package directory is named ``motion-estimation_amd``; load it with
+``__graft_entry__.load_package()`` (registers it as ``motion_estimation_amd``).
+"""
+from . import _lib  # noqa: F401
+from .utils import compute_transform_grids_voxel  # noqa: F401
+from .transform import RigidTransformCudaOptimzied, RigidTransformDerivativeCuda  # noqa: F401

@@ -1,0 +1,93 @@
+"""ctypes binding of libmeamd.so (the C ABI declared in include/motion_estimation_amd.h).
+
+The library is the only compute path: if it is missing or a GPU is absent
+the calls raise; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmeamd.so")
+
+_lock = threading.Lock()
+_lib = None
+
+vp = C.c_void_p
+i32 = C.c_int
+i64 = C.c_longlong
+dp = C.c_void_p  # device pointers are passed as void*
+
+# name -> argtypes (restype int unless noted)
+_SIGS = {
+    "me_version": [],
+    "me_rigid_transform": [vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp, i32, i32, dp, vp],
+    "me_rigid_deriv": [vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp, i32, vp],
+    "me_fft_axis": [vp, i64, vp, i64, i32, vp, vp, i32, i32, vp],
+}
+_OPTIONAL = {}
+
+
+def declared_symbols():
+    """Names of every entry point the Python layer binds."""
+    return sorted(["me_last_error", *_SIGS.keys(), *_OPTIONAL.keys()])
+
+
+def load():
+    """Load libmeamd.so once; raises ImportError if it has not been built."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+        lib = C.CDLL(LIB_PATH)
+        lib.me_last_error.restype = C.c_char_p
+        lib.me_last_error.argtypes = []
+        for name, args in {**_SIGS, **_OPTIONAL}.items():
+            fn = getattr(lib, name)
+            fn.restype = C.c_int
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def call(name, *args):
+    """Invoke an entry point; non-zero status raises RuntimeError(message)."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.me_last_error().decode(errors="replace")
+        raise RuntimeError(f"{name} failed ({rc}): {msg}")
+
+
+def stream(device=None):
+    """Current HIP stream handle (torch's current stream on the device)."""
+    return vp(torch.cuda.current_stream(device).cuda_stream)
+
+
+def ptr(t):
+    """Device (or host) address of a tensor, None for None."""
+    if t is None:
+        return None
+    return vp(t.data_ptr())
+
+
+def int3(vals):
+    return (C.c_int * 3)(*[int(v) for v in vals])
+
+
+def ll3(vals):
+    return (C.c_longlong * 3)(*[int(v) for v in vals])
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("motion_estimation_amd needs a HIP device (MI355X); none is visible")
+    return torch.device("cuda", torch.cuda.current_device())

@@ -1,0 +1,322 @@
+// Line-pass kernel: for every line of a volume along one axis,
+//   forward FFT -> per-(output,input) diagonal phase -> inverse FFT,
+// entirely in registers + LDS, one HBM read of each input and one write of
+// each output.  This is the building block of the FFT-shear rigid transform
+// (transform.py:243-504): every shear step "fft along an axis, multiply by
+// exp(i c r_b k_a), ifft" (transform.py:279-301) is one pass, translation
+// factors (transform.py:308-312) fold into passes along the same axis, and
+// the rotation-derivative product rule (transform.py:433-474) becomes passes
+// with two inputs and/or two outputs.
+//
+// Thread layout: a line of N = R1*R2 points is owned by T = max(R1, R2)
+// threads; step 1 runs R2 threads each holding the R1 points t, t+R2, ...
+// (a register DFT of length R1), one LDS exchange, step 2 runs R1 threads of
+// length-R2 register DFTs.  The inverse retraces the same path, so each thread
+// writes exactly the addresses it read (passes may run in place).
+// A workgroup holds L = 16 lines.  For a strided line axis the 16 lines are
+// consecutive along the unit-stride axis (lanes line-fastest, 128-byte
+// segments); for a unit-stride line axis, lanes run along the line.
+#include "me_device.h"
+#include "me_internal.h"
+
+namespace me {
+
+constexpr int kLinesPerWG = 16;
+
+template <int N>
+struct LineT {
+  static constexpr int R1 = LineFactor<N>::R1;
+  static constexpr int R2 = LineFactor<N>::R2;
+  static constexpr int T = R1 > R2 ? R1 : R2;
+  static constexpr int NT = T * kLinesPerWG;
+};
+
+// Phase vector for one (output, input) pair on the k values owned by a
+// step-2 thread: k = t + R1*k2, k2 = 0..R2-1.
+template <int N, int R1, int R2>
+__device__ __forceinline__ void phase_vec(const PhaseSpec& P, const double* cf, double rb, int t,
+                                          float2 (&ph)[R2]) {
+  const double c = P.ic >= 0 ? (double)P.sc * cf[P.ic] : 0.0;
+  const double d = P.id >= 0 ? (double)P.sd * cf[P.id] : 0.0;
+  const double beta = c * rb + d;  // shift (in samples) scaled so that angle = k_a * beta
+  float2 cur = cis2pi(frac_half(beta * (double)t / (double)N));
+  const float2 step = cis2pi(frac_half(beta * (double)R1 / (double)N));
+  const float2 wrap = cis2pi(frac_half(-beta));
+  float gam = 0.f;
+  if (P.deriv) {
+    const double g = P.ig >= 0 ? (double)P.sg * cf[P.ig] : 0.0;
+    gam = (float)(2.0 * kPi * (g * rb + (double)P.h));
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < R2; ++k2) {
+    const int k = t + R1 * k2;
+    const bool w = 2 * k >= N;
+    float2 e = w ? c_mul(cur, wrap) : cur;
+    if (P.deriv) {
+      const float kap = (float)(k - (w ? N : 0)) / (float)N;
+      e = c_muli(c_scale(e, gam * kap));
+    }
+    ph[k2] = e;
+    cur = c_mul(cur, step);
+  }
+}
+
+template <int N, bool CONTIG, int NIN, int NOUT>
+__global__ void __launch_bounds__(LineT<N>::NT) k_linepass(LineArgs A) {
+  constexpr int R1 = LineT<N>::R1;
+  constexpr int R2 = LineT<N>::R2;
+  constexpr int T = LineT<N>::T;
+  constexpr int L = kLinesPerWG;
+  constexpr int NT = LineT<N>::NT;
+  __shared__ float2 lds[L * R1 * (R2 + 1)];
+  __shared__ float2 tw[N];
+  fill_twiddles<N, NT>(tw);
+
+  const int tid = threadIdx.x;
+  const int t = CONTIG ? (tid % T) : (tid / L);
+  const int l = CONTIG ? (tid / T) : (tid % L);
+  int u, v;
+  bool valid;
+  if (CONTIG) {
+    const long long g = (long long)blockIdx.x * L + l;
+    valid = g < (long long)A.n_u * A.n_v;
+    u = valid ? (int)(g % A.n_u) : 0;
+    v = valid ? (int)(g / A.n_u) : 0;
+  } else {
+    const int tiles_u = (A.n_u + L - 1) / L;
+    v = blockIdx.x / tiles_u;
+    u = (blockIdx.x % tiles_u) * L + l;
+    valid = u < A.n_u;
+    if (!valid) u = 0;
+  }
+  const long long line_off = (long long)u * A.st_u + (long long)v * A.st_v;
+  auto lidx = [&](int k1, int n2) -> int {
+    return CONTIG ? (l * (R1 * (R2 + 1)) + k1 * (R2 + 1) + n2) : (k1 * ((R2 + 1) * L) + n2 * L + l);
+  };
+  const double ru = A.r_u ? A.r_u[u] : 0.0;
+  const double rv = A.r_v ? A.r_v[v] : 0.0;
+  __syncthreads();
+
+  const float inv_scale = A.mode == 0 ? 1.0f / (float)N : rsqrtf((float)N);
+  const int b_begin = A.reduce ? 0 : (int)blockIdx.y;
+  const int b_end = A.reduce ? A.nbatch : (int)blockIdx.y + 1;
+  float2 red[R1];
+#pragma unroll
+  for (int i = 0; i < R1; ++i) red[i] = make_float2(0.f, 0.f);
+
+  for (int b = b_begin; b < b_end; ++b) {
+    const double* cf = A.coef ? A.coef + (long long)b * A.ncoef : nullptr;
+    // ---------------- forward transforms of every input
+    float2 X[NIN][R2];
+#pragma unroll
+    for (int i = 0; i < NIN; ++i) {
+      const float2* src = A.in[i] + (long long)b * A.in_bs[i] + line_off;
+      if (A.mode == 2) {  // input already in the frequency domain, natural order
+        if (t < R1) {
+#pragma unroll
+          for (int k2 = 0; k2 < R2; ++k2)
+            X[i][k2] = valid ? src[(long long)(t + R1 * k2) * A.st_a] : make_float2(0.f, 0.f);
+        }
+        continue;
+      }
+      if (t < R2) {
+        float2 a[R1];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1)
+          a[n1] = valid ? src[(long long)(R2 * n1 + t) * A.st_a] : make_float2(0.f, 0.f);
+        RDFT<R1, -1>::run(a);
+#pragma unroll
+        for (int k1 = 1; k1 < R1; ++k1) a[k1] = c_mul(a[k1], tw[(t * k1) % N]);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) lds[lidx(k1, t)] = a[k1];
+      }
+      __syncthreads();
+      if (t < R1) {
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) X[i][n2] = lds[lidx(t, n2)];
+        RDFT<R2, -1>::run(X[i]);
+      }
+      __syncthreads();
+    }
+
+    // ---------------- per output: combine inputs with their phases, inverse
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o) {
+      float2 acc[R2];
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) acc[k2] = make_float2(0.f, 0.f);
+      if (t < R1) {
+#pragma unroll
+        for (int i = 0; i < NIN; ++i) {
+          const PhaseSpec& P = A.ph[o][i];
+          if (!P.active) continue;
+          if (P.ic < 0 && P.id < 0 && !P.deriv) {  // unit phase
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) acc[k2] = c_add(acc[k2], X[i][k2]);
+          } else {
+            float2 ph[R2];
+            const double rb = P.slot == 0 ? ru : (P.slot == 1 ? rv : 0.0);
+            phase_vec<N, R1, R2>(P, cf, rb, t, ph);
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) acc[k2] = c_fma(X[i][k2], ph[k2], acc[k2]);
+          }
+        }
+      }
+      float2* dst = A.out[o] + (long long)(A.reduce ? 0 : b) * A.out_bs[o] + line_off;
+      if (A.mode == 1) {  // forward only: store natural-order spectrum
+        if (t < R1 && valid) {
+#pragma unroll
+          for (int k2 = 0; k2 < R2; ++k2) {
+            const long long pos = (long long)(t + R1 * k2) * A.st_a;
+            float2 val = c_scale(acc[k2], inv_scale);
+            if (A.accumulate) val = c_add(val, dst[pos]);
+            dst[pos] = val;
+          }
+        }
+        continue;
+      }
+      if (t < R1) {
+        RDFT<R2, +1>::run(acc);
+#pragma unroll
+        for (int n2 = 1; n2 < R2; ++n2) acc[n2] = c_mulc(acc[n2], tw[(n2 * t) % N]);
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) lds[lidx(t, n2)] = acc[n2];
+      }
+      __syncthreads();
+      if (t < R2) {
+        float2 a[R1];
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) a[k1] = lds[lidx(k1, t)];
+        RDFT<R1, +1>::run(a);
+        if (A.reduce) {
+#pragma unroll
+          for (int n1 = 0; n1 < R1; ++n1) red[n1] = c_add(red[n1], a[n1]);
+        } else if (valid) {
+#pragma unroll
+          for (int n1 = 0; n1 < R1; ++n1) {
+            const long long pos = (long long)(R2 * n1 + t) * A.st_a;
+            float2 val = c_scale(a[n1], inv_scale);
+            if (A.mask && !A.mask[(long long)b * A.mask_bs + line_off + pos]) val = make_float2(0.f, 0.f);
+            if (A.accumulate) val = c_add(val, dst[pos]);
+            dst[pos] = val;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (A.reduce && t < R2 && valid) {
+    float2* dst = A.out[0] + line_off;
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) {
+      const long long pos = (long long)(R2 * n1 + t) * A.st_a;
+      float2 val = c_scale(red[n1], inv_scale);
+      if (A.mask && !A.mask[line_off + pos]) val = make_float2(0.f, 0.f);
+      if (A.accumulate) val = c_add(val, dst[pos]);
+      dst[pos] = val;
+    }
+  }
+}
+
+namespace {
+template <bool CONTIG, int NIN, int NOUT>
+struct LineLaunch {
+  const LineArgs* args;
+  dim3 grid;
+  hipStream_t stream;
+  template <int N>
+  void operator()() {
+    hipLaunchKernelGGL((k_linepass<N, CONTIG, NIN, NOUT>), grid, dim3(LineT<N>::NT), 0, stream, *args);
+  }
+};
+
+template <bool CONTIG, int NIN, int NOUT>
+bool launch_n(int n, const LineArgs& a, dim3 grid, hipStream_t s) {
+  LineLaunch<CONTIG, NIN, NOUT> f{&a, grid, s};
+  return dispatch_line_length(n, f);
+}
+
+template <bool CONTIG>
+bool launch_io(int n, int nin, int nout, const LineArgs& a, dim3 grid, hipStream_t s) {
+  if (nin == 1 && nout == 1) return launch_n<CONTIG, 1, 1>(n, a, grid, s);
+  if (nin == 1 && nout == 2) return launch_n<CONTIG, 1, 2>(n, a, grid, s);
+  if (nin == 2 && nout == 2) return launch_n<CONTIG, 2, 2>(n, a, grid, s);
+  if (nin == 2 && nout == 1) return launch_n<CONTIG, 2, 1>(n, a, grid, s);
+  return false;
+}
+}  // namespace
+
+int launch_linepass(const VolGeom& g, int axis, int nin, int nout, LineArgs A, const HostPhase ph[2][2],
+                    hipStream_t stream) {
+  ME_REQUIRE(axis >= 0 && axis < 3, "line pass: axis out of range");
+  ME_REQUIRE(nin >= 1 && nin <= 2 && nout >= 1 && nout <= 2, "line pass: 1 or 2 inputs/outputs");
+  ME_REQUIRE(!(A.reduce && nout != 1), "line pass: batch reduction needs a single output");
+  ME_REQUIRE(A.nbatch >= 1, "line pass: empty batch");
+  const int o1 = axis == 0 ? 1 : 0;
+  const int o2 = axis == 2 ? 1 : 2;
+  int au, av;
+  bool contig;
+  if (g.st[axis] == 1) {
+    contig = true;
+    au = o1;
+    av = o2;
+    if (g.st[o2] < g.st[o1]) {
+      au = o2;
+      av = o1;
+    }
+  } else {
+    contig = false;
+    if (g.st[o1] == 1) {
+      au = o1;
+      av = o2;
+    } else if (g.st[o2] == 1) {
+      au = o2;
+      av = o1;
+    } else {
+      au = g.st[o1] <= g.st[o2] ? o1 : o2;
+      av = au == o1 ? o2 : o1;
+    }
+  }
+  A.st_a = g.st[axis];
+  A.st_u = g.st[au];
+  A.st_v = g.st[av];
+  A.n_u = g.n[au];
+  A.n_v = g.n[av];
+  A.r_u = g.r[au];
+  A.r_v = g.r[av];
+  for (int o = 0; o < 2; ++o)
+    for (int i = 0; i < 2; ++i) {
+      const HostPhase& h = ph[o][i];
+      PhaseSpec& p = A.ph[o][i];
+      p.active = (o < nout && i < nin && h.active) ? 1 : 0;
+      p.ic = h.ic;
+      p.sc = h.sc;
+      p.id = h.id;
+      p.sd = h.sd;
+      p.ig = h.ig;
+      p.sg = h.sg;
+      p.h = h.h;
+      p.deriv = h.deriv;
+      p.slot = -1;
+      if (p.active && h.b_axis >= 0) {
+        ME_REQUIRE(h.b_axis != axis, "line pass: phase cannot depend on the line axis");
+        p.slot = h.b_axis == au ? 0 : 1;
+        ME_REQUIRE(g.r[h.b_axis] != nullptr, "line pass: missing grid vector");
+      }
+      if (p.active && (p.ic >= 0 || p.id >= 0 || p.ig >= 0))
+        ME_REQUIRE(A.coef != nullptr, "line pass: missing coefficient table");
+    }
+  const int L = kLinesPerWG;
+  long long nblk = contig ? ((long long)A.n_u * A.n_v + L - 1) / L : (long long)((A.n_u + L - 1) / L) * A.n_v;
+  ME_REQUIRE(nblk > 0 && nblk < (1LL << 31), "line pass: grid too large");
+  dim3 grid((unsigned)nblk, A.reduce ? 1u : (unsigned)A.nbatch);
+  ME_REQUIRE(A.reduce || A.nbatch <= 65535, "line pass: batch too large");
+  const bool ok = contig ? launch_io<true>(g.n[axis], nin, nout, A, grid, stream)
+                         : launch_io<false>(g.n[axis], nin, nout, A, grid, stream);
+  ME_REQUIRE(ok, "line pass: unsupported line length " + std::to_string(g.n[axis]) +
+                     " (supported: 8,12,16,24,32,48,64,96,128,192,256,384,512)");
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace me
