@@ -35,6 +35,7 @@ extern "C" {
 
 #define ME_FLAG_SUM_BATCH 1 /* adjoint: sum the batch into one output volume */
 #define ME_FLAG_MASK 2      /* multiply the (summed) output by a uint8 mask */
+#define ME_FLAG_ACCUMULATE 4 /* add the (summed) output to y instead of overwriting it */
 
 /* Message of the last failed call on this host thread. */
 const char* me_last_error(void);
@@ -70,6 +71,122 @@ int me_rigid_deriv(const void* x, long long x_bstride, void* y, long long y_bstr
  * calls used to prepare data (joint_recon.py:63-68, pyramid.py:65). */
 int me_fft_axis(const void* x, long long x_bstride, void* y, long long y_bstride, int nbatch, const int* n,
                 const long long* st, int axis, int inverse, hipStream_t stream);
+
+
+/* ---------------------------------------------------------------- encoding
+ * Sample plan of a set of shots (host-built from the shot masks
+ * (S, N0, N1, 1), simulation.py:396-402): samples (k0, k1) of every shot in one
+ * global list, each shot's run contiguous and sorted by (k1, k0);
+ * shot_off/shot_cnt give each shot's run; colptr holds, per shot, N1+1
+ * column pointers relative to its run.  Gathered hybrid data of a shot live
+ * at Y + shot_off*N2*C, laid out [x2][coil][sample].
+ * Images and maps are in plane-major layout (see above); maps are C volumes. */
+
+/* Size (doubles) of the per-workgroup partials workspace of me_encode_forward
+ * for one shot; multiply by nbatch. */
+long long me_encode_partials_size(int n2, int ncoils, int max_cnt, int n1);
+
+/* Gathered forward encoding of a batch of shots:
+ *   Y_s = M_s F_01 (S_c u_s)      (transform_estimation.py:207-210 without the
+ *   readout FFT, which is applied once to the measured data)
+ * u: image per shot at u_bstride (0: one image for all shots).  If y_meas is
+ * given, Y - y_meas is produced (the residual).  y (nullable) receives the
+ * values; norms (nullable, needs partials) receives ||Y - y_meas||^2 per shot
+ * in FP64.  max_cnt: largest shot sample count in the batch. */
+int me_encode_forward(const void* u, long long u_bstride, const void* maps, int ncoils, int nbatch, const int* n,
+                      const int* samp_k0, const int* samp_k1, const int* shot_off, const int* shot_cnt, int max_cnt,
+                      void* y, const void* y_meas, double* partials, double* norms, hipStream_t stream);
+
+/* Adjoint of me_encode_forward for a batch of shots:
+ *   out_s = sum_c conj(S_c) F_01^H (M_s Y_s)     (image_estimation.py:94-102,
+ *   before T^H).  out at out_bstride per shot (plane-major). */
+int me_encode_adjoint(const void* y, const void* maps, int ncoils, int nbatch, const int* n, const int* samp_k0,
+                      const int* shot_off, const int* shot_cnt, const int* colptr, void* out, long long out_bstride,
+                      hipStream_t stream);
+
+/* y[c] = scale * x[(c + offset) mod x_dims] over 4-D index spaces (coil, x0,
+ * x1, x2) with arbitrary strides, y_dims <= x_dims.  Used for the reference's
+ * ifftshift/fftshift (joint_recon.py:63-65, 106), SigPy-style centred crops
+ * (pyramid.py:65-67) and layout changes. */
+int me_permute_shift(const void* x, const long long* x_strides, const int* x_dims, void* y,
+                     const long long* y_strides, const int* y_dims, const int* offset, float scale,
+                     hipStream_t stream);
+
+/* P = 1/(sum_c |S_c|^2 + eps), M = sum_c |S_c|^2 > thr (joint_recon.py:83-85);
+ * either output may be null. */
+int me_coil_ssq(const void* maps, int ncoils, long long vol, float eps, float thr, float* P, unsigned char* M,
+                hipStream_t stream);
+
+/* Gather measured hybrid data: Ymeas[shot_off*N2*C + (x2*C + c)*cnt + w] =
+ * scale * ksp_h[c][k0_w][k1_w][x2], ksp_h in the reference C order after the
+ * readout inverse FFT. */
+int me_gather_hybrid(const void* ksp_h, int ncoils, const int* n, const int* samp_k0, const int* samp_k1,
+                     const int* shot_off, const int* shot_cnt, int nshots, long long total_samples, float scale,
+                     void* ymeas, hipStream_t stream);
+
+/* ksp_h[c][k0_w][k1_w][x2] += Y[...]: scatter of gathered hybrid data back to
+ * dense C-order hybrid k-space (atomic adds).  Used to synthesise the
+ * motion-corrupted k-space, sum_s M_s F S T_s x (simulation.py:65-85). */
+int me_scatter_hybrid(const void* y, int ncoils, const int* n, const int* samp_k0, const int* samp_k1,
+                      const int* shot_off, const int* shot_cnt, int nshots, long long total_samples, void* ksp_h,
+                      hipStream_t stream);
+
+/* *out_bits = max(*out_bits, bits of max |x_i|) (atomic; zero it first). */
+int me_max_abs(const void* x, long long n, unsigned int* out_bits, hipStream_t stream);
+
+/* ---------------------------------------------------------------- CG / LM
+ * CG state: double[8] = {rsold, rsnew, pAp, alpha, beta, done, iters, -}. */
+
+/* Number of doubles in the partials workspace of the vector kernels. */
+int me_vec_partials_size(void);
+
+/* *out = Re sum_i conj(a_i) b_i (w_i if w) in FP64 (cupy.vdot(...).real). */
+int me_dot_re(const void* a, const void* b, const float* w, long long n, double* partials, double* out,
+              hipStream_t stream);
+
+/* r = b - Ap; p = P r; rsold = Re<r, P r>; done = 0 (image_estimation.py:133-138). */
+int me_cg_init(const void* b, const void* Ap, void* r, void* p, const float* P, long long n, double* state,
+               double* partials, hipStream_t stream);
+
+/* One PCG update after Ap = A p (image_estimation.py:143-163): pAp, alpha,
+ * x += alpha p, r -= alpha Ap, rsnew = Re<r, P r>, the tolerance test, beta,
+ * p = P r + beta p.  No-op once the tolerance test has fired. */
+int me_cg_step(void* x, void* r, void* p, const void* Ap, const float* P, long long n, double tol, double* state,
+               double* partials, hipStream_t stream);
+
+/* x *= mask (uint8) (joint_recon.py:100). */
+int me_mask_mul(void* x, const unsigned char* mask, long long n, hipStream_t stream);
+
+/* Doubles of LM gram partials for nbatch shots. */
+long long me_lm_partials_size(int nbatch);
+
+/* Per shot: stats[28] = {||r||^2, Re J_p^H r (6), Re J_i^H J_j (i<=j, 21)} over
+ * its gathered samples in FP64 (transform_estimation.py:212-233).  r and the
+ * columns use the gathered layout; column p at J + p*jstride; per_sample =
+ * N2*C values per sample. */
+int me_lm_gram(const void* r, const void* J, long long jstride, const int* shot_off, const int* shot_cnt,
+               int per_sample, int nbatch, double* partials, double* stats, hipStream_t stream);
+
+/* (Gram + damp I) delta = g, cand = theta - delta (transform_estimation.py:236, 290). */
+int me_lm_solve(const double* stats, const double* damp, const double* theta, int theta_stride, double* cand,
+                double* delta, int nbatch, hipStream_t stream);
+
+/* Accept iff e1 < e0: theta = cand, damp = max(damp/5, 1e-4), flags set if
+ * |theta - cand| < tol; else damp = min(1.5 damp, 1e16)
+ * (transform_estimation.py:303-309).  accepted (nullable) records decisions. */
+int me_lm_accept(const double* stats, const double* e1, double* theta, int theta_stride, const double* cand,
+                 double* damp, unsigned char* flags, unsigned char* accepted, double tol, int nbatch,
+                 hipStream_t stream);
+
+/* mean[6] = mean over shots of theta; theta -= mean (transform_estimation.py:312-315). */
+int me_lm_mean(double* theta, int theta_stride, int nshots, double* mean, hipStream_t stream);
+
+/* ---------------------------------------------------------------- host
+ * DISORDER order (simulation.py:201-245), bit-exact with the reference's
+ * stdlib random.Random(seed).shuffle.  Host function, no device work. */
+int me_disorder_order(int max_line_number, int max_partition_number, int pat_lines_to_measure, int block_lin,
+                      int block_par, unsigned long long seed, int* out_line, int* out_part, long long capacity,
+                      long long* out_count, int* out_L, int* out_P);
 
 #ifdef __cplusplus
 }

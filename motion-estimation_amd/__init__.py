@@ -5,6 +5,11 @@ include/motion_estimation_amd.h):
 
 * transform: RigidTransformCudaOptimzied, RigidTransformDerivativeCuda
 * utils: compute_transform_grids_voxel
+* image_estimation: estimate_image_cg (CG-SENSE)
+* transform_estimation: estimate_transform (per-shot Levenberg-Marquardt)
+* joint_recon: JointRecon
+* pyramid: pyramid_reconstruction
+* simulation: DISORDER sampling, corrupted k-space, synthetic inputs
 
 The package directory is named ``motion-estimation_amd``; load it with
 ``__graft_entry__.load_package()`` (registers it as ``motion_estimation_amd``).
@@ -12,3 +17,8 @@ The package directory is named ``motion-estimation_amd``; load it with
 from . import _lib  # noqa: F401
 from .utils import compute_transform_grids_voxel  # noqa: F401
 from .transform import RigidTransformCudaOptimzied, RigidTransformDerivativeCuda  # noqa: F401
+from . import encoding, simulation  # noqa: F401
+from .image_estimation import estimate_image_cg  # noqa: F401
+from .transform_estimation import estimate_transform  # noqa: F401
+from .joint_recon import JointRecon  # noqa: F401
+from .pyramid import pyramid_reconstruction  # noqa: F401

@@ -29,13 +29,35 @@ _SIGS = {
     "me_rigid_transform": [vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp, i32, i32, dp, vp],
     "me_rigid_deriv": [vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp, i32, vp],
     "me_fft_axis": [vp, i64, vp, i64, i32, vp, vp, i32, i32, vp],
+    "me_encode_forward": [vp, i64, vp, i32, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp],
+    "me_encode_adjoint": [vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, i64, vp],
+    "me_permute_shift": [vp, vp, vp, vp, vp, vp, vp, C.c_float, vp],
+    "me_coil_ssq": [vp, i32, i64, C.c_float, C.c_float, vp, vp, vp],
+    "me_gather_hybrid": [vp, i32, vp, vp, vp, vp, vp, i32, i64, C.c_float, vp, vp],
+    "me_max_abs": [vp, i64, vp, vp],
+    "me_scatter_hybrid": [vp, i32, vp, vp, vp, vp, vp, i32, i64, vp, vp],
+    "me_dot_re": [vp, vp, vp, i64, vp, vp, vp],
+    "me_cg_init": [vp, vp, vp, vp, vp, i64, vp, vp, vp],
+    "me_cg_step": [vp, vp, vp, vp, vp, i64, C.c_double, vp, vp, vp],
+    "me_mask_mul": [vp, vp, i64, vp],
+    "me_lm_gram": [vp, vp, i64, vp, vp, i32, i32, vp, vp, vp],
+    "me_lm_solve": [vp, vp, vp, i32, vp, vp, i32, vp],
+    "me_lm_accept": [vp, vp, vp, i32, vp, vp, vp, vp, C.c_double, i32, vp],
+    "me_lm_mean": [vp, i32, i32, vp, vp],
+    "me_disorder_order": [i32, i32, i32, i32, i32, C.c_ulonglong, vp, vp, i64, vp, vp, vp],
+}
+# entry points returning sizes rather than a status
+_SIZES = {
+    "me_encode_partials_size": ([i32, i32, i32, i32], C.c_longlong),
+    "me_vec_partials_size": ([], C.c_int),
+    "me_lm_partials_size": ([i32], C.c_longlong),
 }
 _OPTIONAL = {}
 
 
 def declared_symbols():
     """Names of every entry point the Python layer binds."""
-    return sorted(["me_last_error", *_SIGS.keys(), *_OPTIONAL.keys()])
+    return sorted(["me_last_error", *_SIGS.keys(), *_SIZES.keys(), *_OPTIONAL.keys()])
 
 
 def load():
@@ -54,6 +76,10 @@ def load():
             fn = getattr(lib, name)
             fn.restype = C.c_int
             fn.argtypes = args
+        for name, (args, res) in _SIZES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
         _lib = lib
         return lib
 
@@ -65,6 +91,11 @@ def call(name, *args):
     if rc != 0:
         msg = lib.me_last_error().decode(errors="replace")
         raise RuntimeError(f"{name} failed ({rc}): {msg}")
+
+
+def size(name, *args):
+    """Value of a size-query entry point."""
+    return getattr(load(), name)(*args)
 
 
 def stream(device=None):

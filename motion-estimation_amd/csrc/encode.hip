@@ -1,0 +1,623 @@
+// Per-shot SENSE encoding E_s = M_s F C and its adjoint, in the hybrid
+// (k0, k1, x2) space.
+//
+// The reference computes, per shot s and coil c (image_estimation.py:94-128,
+// transform_estimation.py:207-210, joint_recon.py:428-429)
+//     E_s u = M_s * fftn(S_c * u)   over all three axes, with a dense boolean
+// mask M_s(k0, k1) broadcast along the readout axis 2.  Because the mask does
+// not depend on k2 and the 3-D FFT is orthonormal and separable, the readout
+// FFT can be moved onto the measured data once (hybrid space
+// y_h = ifft_2(y), a norm- and inner-product-preserving change of basis), so
+// only the 2-D transform over (x0, x1) and only the sampled (k0, k1) lines are
+// ever computed:
+//     Y_s[x2][c][w] = (N0 N1)^-1/2 sum_{x0,x1} e^{-2 pi i (k0 x0/N0 + k1 x1/N1)} S_c u
+// for the samples w = (k0, k1) of shot s.  Forward: a workgroup owns one
+// readout plane x2 and 16 coils, streams the N0 rows of S_c * u through a
+// row FFT along axis 1 (registers + LDS) and accumulates the sparse DFT along
+// axis 0 for its samples in registers, so each map and image element is read
+// from HBM once and only the samples are written.  Adjoint: a workgroup owns
+// a block of rows of one plane, stages a coil group's samples in LDS, builds
+// each row's spectrum by a column-sorted scatter, inverse row FFT, multiplies
+// by conj(S_c), and sums the coils in registers.
+//
+// Volumes (image u, maps S_c) are in plane-major layout: element (x0, x1, x2)
+// at (x2*N0 + x0)*N1 + x1.  Samples of a shot are sorted by (k1, k0).
+#include "me_device.h"
+#include "me_internal.h"
+#include "../../include/motion_estimation_amd.h"
+
+namespace me {
+
+constexpr int kCoilsPerWG = 16;
+constexpr int kMaxCol = 512;  // maximum N0 (column length, LDS twiddle table)
+constexpr int kAdjChunk = 512;  // samples per coil staged in LDS by the adjoint
+
+template <int NR>
+struct RowT {
+  static constexpr int R1 = LineFactor<NR>::R1;
+  static constexpr int R2 = LineFactor<NR>::R2;
+  static constexpr int T = R1 > R2 ? R1 : R2;
+  static constexpr int NT = T * kCoilsPerWG;
+};
+
+struct EncArgs {
+  const float2* u;   // image batch (plane-major)
+  long long u_bs;    // batch stride (0: same image for every shot)
+  const float2* maps;
+  int ncoils;
+  int n0, n1, n2;
+  const int* sk0;   // sample k0 (global list)
+  const int* sk1;   // sample k1
+  const int* off;   // per batch entry: first sample in the global list
+  const int* cnt;   // per batch entry: number of samples
+  const int* colptr;  // per batch entry: (n1+1) column pointers, relative to off
+  float2* y;          // gathered output (may be null when only the norm is wanted)
+  const float2* meas;  // gathered measured data to subtract (null: none)
+  double* partial;     // per-workgroup FP64 partial sums of |y - meas|^2 (null: none)
+  int ncg;             // coil groups
+  int nchunk;          // sample chunks per shot
+  int blocks_per_shot;
+  float2* out;         // adjoint output batch (plane-major)
+  long long out_bs;
+  int rows_per_wg;
+  int max_cnt;
+};
+
+__device__ __forceinline__ double block_sum_f64(double v, double* sh) {
+  // wave64 shuffle reduction, then one value per wave through LDS
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) s += sh[i];
+  }
+  return s;
+}
+
+template <int NR, int SPT>
+__global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
+  constexpr int R1 = RowT<NR>::R1, R2 = RowT<NR>::R2, T = RowT<NR>::T, NT = RowT<NR>::NT;
+  constexpr int G = kCoilsPerWG;
+  __shared__ float2 xch[G * R1 * (R2 + 1)];
+  __shared__ float2 zrow[G * NR];
+  __shared__ float2 twr[NR];
+  __shared__ float2 twc[kMaxCol];
+  __shared__ double red[16];
+
+  const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
+  const int b = blockIdx.y;
+  int bx = blockIdx.x;
+  const int x2 = bx % n2;
+  bx /= n2;
+  const int cg = bx % A.ncg;
+  const int chunk = bx / A.ncg;
+  const int tid = threadIdx.x;
+  const int t = tid % T, g = tid / T;
+  const int c = cg * G + g;
+  const bool cvalid = c < A.ncoils;
+  const int cnt = A.cnt[b], off = A.off[b];
+
+  fill_twiddles<NR, NT>(twr);
+  for (int j = tid; j < n0; j += NT) {
+    float s, co;
+    const int jj = (2 * j > n0) ? j - n0 : j;
+    sincospif(-2.0f * (float)jj / (float)n0, &s, &co);
+    twc[j] = make_float2(co, s);
+  }
+  int mk0[SPT], mk1[SPT];
+  bool mv[SPT];
+#pragma unroll
+  for (int j = 0; j < SPT; ++j) {
+    const int w = chunk * (SPT * NT) + j * NT + tid;
+    mv[j] = w < cnt;
+    mk0[j] = mv[j] ? A.sk0[off + w] : 0;
+    mk1[j] = mv[j] ? A.sk1[off + w] : 0;
+  }
+  float2 acc[SPT][G];
+#pragma unroll
+  for (int j = 0; j < SPT; ++j)
+#pragma unroll
+    for (int q = 0; q < G; ++q) acc[j][q] = make_float2(0.f, 0.f);
+
+  const long long plane = (long long)n0 * n1;
+  const long long vol = plane * n2;
+  const float2* up = A.u + (long long)b * A.u_bs + (long long)x2 * plane;
+  const float2* sp = A.maps + (long long)(cvalid ? c : 0) * vol + (long long)x2 * plane;
+  auto lidx = [&](int k1, int m2) { return g * (R1 * (R2 + 1)) + k1 * (R2 + 1) + m2; };
+  __syncthreads();
+
+  for (int x0 = 0; x0 < n0; ++x0) {
+    if (t < R2) {
+      float2 a[R1];
+      const int ro = x0 * n1;
+#pragma unroll
+      for (int m1 = 0; m1 < R1; ++m1) {
+        const int pos = ro + R2 * m1 + t;
+        a[m1] = cvalid ? c_mul(sp[pos], up[pos]) : make_float2(0.f, 0.f);
+      }
+      RDFT<R1, -1>::run(a);
+#pragma unroll
+      for (int k1 = 1; k1 < R1; ++k1) a[k1] = c_mul(a[k1], twr[(t * k1) % NR]);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) xch[lidx(k1, t)] = a[k1];
+    }
+    __syncthreads();
+    if (t < R1) {
+      float2 X[R2];
+#pragma unroll
+      for (int m2 = 0; m2 < R2; ++m2) X[m2] = xch[lidx(t, m2)];
+      RDFT<R2, -1>::run(X);
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) zrow[g * NR + t + R1 * k2] = X[k2];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+      if (mv[j]) {
+        const float2 w = twc[(mk0[j] * x0) % n0];
+#pragma unroll
+        for (int q = 0; q < G; ++q) acc[j][q] = c_fma(w, zrow[q * NR + mk1[j]], acc[j][q]);
+      }
+    }
+  }
+
+  const float scale = rsqrtf((float)n0 * (float)n1);
+  double local = 0.0;
+  const long long ybase = (long long)off * n2 * A.ncoils;
+#pragma unroll
+  for (int j = 0; j < SPT; ++j) {
+    if (!mv[j]) continue;
+    const int w = chunk * (SPT * NT) + j * NT + tid;
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      const int cc = cg * G + q;
+      if (cc >= A.ncoils) continue;
+      const long long idx = ybase + ((long long)x2 * A.ncoils + cc) * cnt + w;
+      float2 val = c_scale(acc[j][q], scale);
+      if (A.meas) val = c_sub(val, A.meas[idx]);
+      if (A.y) A.y[idx] = val;
+      local += (double)val.x * val.x + (double)val.y * val.y;
+    }
+  }
+  if (A.partial) {
+    const double s = block_sum_f64(local, red);
+    if (tid == 0) A.partial[(long long)b * A.blocks_per_shot + blockIdx.x] = s;
+  }
+}
+
+template <int NR>
+__global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
+  constexpr int R1 = RowT<NR>::R1, R2 = RowT<NR>::R2, T = RowT<NR>::T, NT = RowT<NR>::NT;
+  constexpr int G = kCoilsPerWG;
+  constexpr int RB = 16;  // rows per workgroup (output accumulators per thread)
+  extern __shared__ float2 dyn[];  // [G][chunk] samples of the coil group
+  __shared__ float2 xch[G * R1 * (R2 + 1)];
+  __shared__ float2 twr[NR];
+  __shared__ float2 twc[kMaxCol];
+  __shared__ int cp[NR + 1];
+  __shared__ int sk0l[kAdjChunk];
+
+  const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
+  const int b = blockIdx.y;
+  const int nrb = (n0 + RB - 1) / RB;
+  const int x2 = blockIdx.x % n2;
+  const int rb = blockIdx.x / n2;
+  const int row0 = rb * RB;
+  const int tid = threadIdx.x;
+  const int t = tid % T, g = tid / T;
+  const int cnt = A.cnt[b], off = A.off[b];
+  const int chunkmax = kAdjChunk;  // LDS capacity in samples per coil
+
+  fill_twiddles<NR, NT>(twr);
+  for (int j = tid; j < n0; j += NT) {
+    float s, co;
+    const int jj = (2 * j > n0) ? j - n0 : j;
+    sincospif(-2.0f * (float)jj / (float)n0, &s, &co);
+    twc[j] = make_float2(co, s);
+  }
+  for (int j = tid; j <= n1; j += NT) cp[j] = A.colptr[(long long)b * (n1 + 1) + j];
+
+  const float scale = rsqrtf((float)n0 * (float)n1);
+  float2* op = A.out + (long long)b * A.out_bs + (long long)x2 * (long long)n0 * n1;
+  const long long plane = (long long)n0 * n1;
+  const long long vol = plane * n2;
+  const long long ybase = (long long)off * n2 * A.ncoils;
+  auto lidx = [&](int k1, int m2) { return g * (R1 * (R2 + 1)) + k1 * (R2 + 1) + m2; };
+  (void)nrb;
+
+  for (int cg = 0; cg < A.ncg; ++cg) {
+    const int c = cg * G + g;
+    const bool cvalid = c < A.ncoils;
+    const float2* sp = A.maps + (long long)(cvalid ? c : 0) * vol + (long long)x2 * plane;
+    for (int s0 = 0; s0 < cnt; s0 += chunkmax) {
+      const int s1 = min(cnt, s0 + chunkmax);
+      __syncthreads();  // previous users of dyn/xch are done
+      for (int j = tid; j < G * chunkmax; j += NT) {
+        const int q = j / chunkmax, w = s0 + j % chunkmax;
+        const int cc = cg * G + q;
+        dyn[j] = (cc < A.ncoils && w < s1) ? A.y[ybase + ((long long)x2 * A.ncoils + cc) * cnt + w]
+                                            : make_float2(0.f, 0.f);
+      }
+      for (int j = tid; j < s1 - s0; j += NT) sk0l[j] = A.sk0[off + s0 + j];
+      __syncthreads();
+      for (int r = 0; r < RB; ++r) {
+        const int x0 = row0 + r;
+        if (x0 >= n0) break;
+        if (t < R2) {
+          float2 a[R1];
+#pragma unroll
+          for (int m1 = 0; m1 < R1; ++m1) {
+            const int k1 = R2 * m1 + t;
+            float2 z = make_float2(0.f, 0.f);
+            const int w0 = max(cp[k1], s0), w1 = min(cp[k1 + 1], s1);
+            for (int w = w0; w < w1; ++w) {
+              const float2 tw = twc[(sk0l[w - s0] * x0) % n0];
+              z = c_fmac(tw, dyn[g * chunkmax + (w - s0)], z);  // conj(e^{-i..}) * Y
+            }
+            a[m1] = z;
+          }
+          RDFT<R1, +1>::run(a);
+#pragma unroll
+          for (int k1 = 1; k1 < R1; ++k1) a[k1] = c_mulc(a[k1], twr[(t * k1) % NR]);
+#pragma unroll
+          for (int k1 = 0; k1 < R1; ++k1) xch[lidx(k1, t)] = a[k1];
+        }
+        __syncthreads();
+        float2 X[R2];
+        if (t < R1) {
+#pragma unroll
+          for (int m2 = 0; m2 < R2; ++m2) X[m2] = xch[lidx(t, m2)];
+          RDFT<R2, +1>::run(X);
+        }
+        __syncthreads();
+        // xch reused as the per-coil product buffer [G][NR]
+        if (t < R1) {
+#pragma unroll
+          for (int k2 = 0; k2 < R2; ++k2) {
+            const int pos = t + R1 * k2;
+            xch[g * NR + pos] = cvalid ? c_mulc(X[k2], sp[x0 * n1 + pos]) : make_float2(0.f, 0.f);
+          }
+        }
+        __syncthreads();
+        if (tid < NR) {  // coil sum; the first contribution writes, later ones add
+          float2 s = make_float2(0.f, 0.f);
+#pragma unroll
+          for (int q = 0; q < G; ++q) s = c_add(s, xch[q * NR + tid]);
+          s = c_scale(s, scale);
+          if (cg > 0 || s0 > 0) s = c_add(s, op[x0 * n1 + tid]);
+          op[x0 * n1 + tid] = s;
+        }
+        __syncthreads();
+      }
+    }
+  }
+  if (cnt == 0 && tid < NR) {  // a shot without samples contributes zero
+    for (int r = 0; r < RB; ++r) {
+      const int x0 = row0 + r;
+      if (x0 < n0) op[x0 * n1 + tid] = make_float2(0.f, 0.f);
+    }
+  }
+}
+
+// Fixed-order sum of per-workgroup partials: out[b] = sum_i partial[b*nper + i]
+__global__ void k_sum_partials(const double* __restrict__ partial, int nper, double* __restrict__ out) {
+  __shared__ double sh[16];
+  double v = 0.0;
+  const double* p = partial + (long long)blockIdx.x * nper;
+  for (int i = threadIdx.x; i < nper; i += blockDim.x) v += p[i];
+  const double s = block_sum_f64(v, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// y[c] = scale * x[(c + off) mod xdims] over 4-D (c, x0, x1, x2) index
+// spaces with arbitrary element strides (ydims <= xdims): covers fftshift /
+// ifftshift (off = n/2 or n - n/2), centred crops (off = n/2 - m/2) and
+// layout changes.  Threads walk y in its memory order (ord[0] slowest).
+struct PermArgs {
+  const float2* x;
+  float2* y;
+  long long xs[4], ys[4];
+  int ydims[4], xdims[4], off[4], ord[4];
+  float scale;
+  long long total;
+};
+__global__ void k_permute_shift(PermArgs P) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.total) return;
+  long long rem = i;
+  int co[4];
+#pragma unroll
+  for (int k = 3; k >= 0; --k) {
+    const int a = P.ord[k];
+    co[a] = (int)(rem % P.ydims[a]);
+    rem /= P.ydims[a];
+  }
+  long long xo = 0, yo = 0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    int s = co[a] + P.off[a];
+    if (s >= P.xdims[a]) s -= P.xdims[a];
+    xo += (long long)s * P.xs[a];
+    yo += (long long)co[a] * P.ys[a];
+  }
+  const float2 v = P.x[xo];
+  P.y[yo] = make_float2(v.x * P.scale, v.y * P.scale);
+}
+
+// Sum over coils of |S_c|^2 (plane-major or any layout; maps coil-contiguous):
+//   P = 1 / (ssq + eps), M = ssq > thr   (joint_recon.py:83-85)
+__global__ void k_coil_ssq(const float2* __restrict__ maps, int ncoils, long long vol, float eps, float thr,
+                           float* __restrict__ P, unsigned char* __restrict__ M) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vol) return;
+  float s = 0.f;
+  for (int c = 0; c < ncoils; ++c) {
+    const float2 v = maps[(long long)c * vol + i];
+    s = fmaf(v.x, v.x, fmaf(v.y, v.y, s));
+  }
+  if (P) P[i] = 1.0f / (s + eps);
+  if (M) M[i] = s > thr ? 1 : 0;
+}
+
+// Ymeas[(off*n2*C) + (x2*C + c)*cnt + w] = scale * ksp_h[c][k0][k1][x2]
+// (ksp_h in reference C order, after the readout inverse FFT).
+__global__ void k_gather_hybrid(const float2* __restrict__ ksp, int ncoils, int n0, int n1, int n2,
+                                const int* __restrict__ sk0, const int* __restrict__ sk1, const int* __restrict__ off,
+                                const int* __restrict__ cnt, int nshots, long long total_samples, float scale,
+                                float2* __restrict__ ymeas) {
+  // one thread per (global sample, coil); loop over x2 (reads contiguous in x2)
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total_samples * ncoils) return;
+  const int c = (int)(i % ncoils);
+  const long long gw = i / ncoils;
+  // locate the shot of this sample (shots are contiguous, off increasing)
+  int lo = 0, hi = nshots - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((long long)off[mid] <= gw) lo = mid; else hi = mid - 1;
+  }
+  const int s = lo;
+  const int w = (int)(gw - off[s]);
+  const long long src = (((long long)c * n0 + sk0[gw]) * n1 + sk1[gw]) * n2;
+  const long long dbase = (long long)off[s] * n2 * ncoils;
+  for (int x2 = 0; x2 < n2; ++x2) {
+    const float2 v = ksp[src + x2];
+    ymeas[dbase + ((long long)x2 * ncoils + c) * cnt[s] + w] = make_float2(v.x * scale, v.y * scale);
+  }
+}
+
+// ksp_h[c][k0][k1][x2] += Y[...]: inverse of k_gather_hybrid (atomic adds, so
+// shots may share lines).
+__global__ void k_scatter_hybrid(const float2* __restrict__ y, int ncoils, int n0, int n1, int n2,
+                                 const int* __restrict__ sk0, const int* __restrict__ sk1, const int* __restrict__ off,
+                                 const int* __restrict__ cnt, int nshots, long long total_samples,
+                                 float2* __restrict__ ksp) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total_samples * ncoils) return;
+  const int c = (int)(i % ncoils);
+  const long long gw = i / ncoils;
+  int lo = 0, hi = nshots - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if ((long long)off[mid] <= gw) lo = mid; else hi = mid - 1;
+  }
+  const int s = lo;
+  const int w = (int)(gw - off[s]);
+  const long long dst = (((long long)c * n0 + sk0[gw]) * n1 + sk1[gw]) * n2;
+  const long long sbase = (long long)off[s] * n2 * ncoils;
+  for (int x2 = 0; x2 < n2; ++x2) {
+    const float2 v = y[sbase + ((long long)x2 * ncoils + c) * cnt[s] + w];
+    atomicAdd(&ksp[dst + x2].x, v.x);
+    atomicAdd(&ksp[dst + x2].y, v.y);
+  }
+}
+
+// max |x| -> atomicMax on the float bit pattern (non-negative floats order as ints)
+__global__ void k_maxabs(const float2* __restrict__ x, long long n, unsigned int* __restrict__ out) {
+  float m = 0.f;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const float2 v = x[i];
+    m = fmaxf(m, sqrtf(v.x * v.x + v.y * v.y));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_down(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+namespace {
+template <int SPT>
+struct FwdLaunch {
+  const EncArgs* a;
+  dim3 grid;
+  hipStream_t s;
+  template <int NR>
+  void operator()() {
+    hipLaunchKernelGGL((k_encode_fwd<NR, SPT>), grid, dim3(RowT<NR>::NT), 0, s, *a);
+  }
+};
+struct AdjLaunch {
+  const EncArgs* a;
+  dim3 grid;
+  size_t shmem;
+  hipStream_t s;
+  template <int NR>
+  void operator()() {
+    hipLaunchKernelGGL((k_encode_adj<NR>), grid, dim3(RowT<NR>::NT), shmem, s, *a);
+  }
+};
+struct RowThreads {
+  int* nt;
+  template <int NR>
+  void operator()() { *nt = RowT<NR>::NT; }
+};
+int threads_for_row(int n1) {
+  int nt = 0;
+  RowThreads f{&nt};
+  dispatch_line_length(n1, f);
+  return nt;
+}
+}  // namespace
+
+}  // namespace me
+
+using namespace me;
+
+extern "C" {
+
+long long me_encode_partials_size(int n2, int ncoils, int max_cnt, int n1) {
+  const int nt = threads_for_row(n1);
+  if (nt == 0 || ncoils <= 0 || n2 <= 0) return 0;
+  const int ncg = (ncoils + kCoilsPerWG - 1) / kCoilsPerWG;
+  const int spt = 2;
+  const int nchunk = max_cnt > 0 ? (max_cnt + spt * nt - 1) / (spt * nt) : 1;
+  return (long long)n2 * ncg * nchunk;
+}
+
+int me_encode_forward(const void* u, long long u_bstride, const void* maps, int ncoils, int nbatch, const int* n,
+                      const int* samp_k0, const int* samp_k1, const int* shot_off, const int* shot_cnt, int max_cnt,
+                      void* y, const void* y_meas, double* partials, double* norms, hipStream_t stream) {
+  ME_REQUIRE(u && maps && n && samp_k0 && samp_k1 && shot_off && shot_cnt, "encode_forward: null argument");
+  ME_REQUIRE(nbatch >= 1 && nbatch <= 65535 && ncoils >= 1, "encode_forward: bad batch/coil count");
+  ME_REQUIRE(n[0] <= kMaxCol, "encode_forward: N0 above 512");
+  ME_REQUIRE(!norms || partials, "encode_forward: norms need a partials workspace");
+  const int nt = threads_for_row(n[1]);
+  ME_REQUIRE(nt > 0, "encode_forward: unsupported N1 " + std::to_string(n[1]));
+  EncArgs A{};
+  A.u = (const float2*)u;
+  A.u_bs = u_bstride;
+  A.maps = (const float2*)maps;
+  A.ncoils = ncoils;
+  A.n0 = n[0];
+  A.n1 = n[1];
+  A.n2 = n[2];
+  A.sk0 = samp_k0;
+  A.sk1 = samp_k1;
+  A.off = shot_off;
+  A.cnt = shot_cnt;
+  A.y = (float2*)y;
+  A.meas = (const float2*)y_meas;
+  A.partial = partials;
+  A.ncg = (ncoils + kCoilsPerWG - 1) / kCoilsPerWG;
+  const int spt = 2;
+  A.nchunk = max_cnt > 0 ? (max_cnt + spt * nt - 1) / (spt * nt) : 1;
+  A.blocks_per_shot = n[2] * A.ncg * A.nchunk;
+  dim3 grid((unsigned)A.blocks_per_shot, (unsigned)nbatch);
+  FwdLaunch<2> f{&A, grid, stream};
+  ME_REQUIRE(dispatch_line_length(n[1], f), "encode_forward: unsupported N1");
+  ME_TRY_HIP(hipGetLastError());
+  if (norms) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(nbatch), dim3(256), 0, stream, partials, A.blocks_per_shot, norms);
+    ME_TRY_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+int me_encode_adjoint(const void* y, const void* maps, int ncoils, int nbatch, const int* n, const int* samp_k0,
+                      const int* shot_off, const int* shot_cnt, const int* colptr, void* out, long long out_bstride,
+                      hipStream_t stream) {
+  ME_REQUIRE(y && maps && n && samp_k0 && shot_off && shot_cnt && colptr && out, "encode_adjoint: null argument");
+  ME_REQUIRE(nbatch >= 1 && nbatch <= 65535 && ncoils >= 1, "encode_adjoint: bad batch/coil count");
+  ME_REQUIRE(n[0] <= kMaxCol, "encode_adjoint: N0 above 512");
+  ME_REQUIRE(threads_for_row(n[1]) > 0, "encode_adjoint: unsupported N1 " + std::to_string(n[1]));
+  EncArgs A{};
+  A.y = (float2*)y;
+  A.maps = (const float2*)maps;
+  A.ncoils = ncoils;
+  A.n0 = n[0];
+  A.n1 = n[1];
+  A.n2 = n[2];
+  A.sk0 = samp_k0;
+  A.off = shot_off;
+  A.cnt = shot_cnt;
+  A.colptr = colptr;
+  A.out = (float2*)out;
+  A.out_bs = out_bstride;
+  A.ncg = (ncoils + kCoilsPerWG - 1) / kCoilsPerWG;
+  const size_t shmem = (size_t)kCoilsPerWG * kAdjChunk * sizeof(float2);
+  const int nrb = (n[0] + 15) / 16;
+  dim3 grid((unsigned)(n[2] * nrb), (unsigned)nbatch);
+  AdjLaunch f{&A, grid, shmem, stream};
+  ME_REQUIRE(dispatch_line_length(n[1], f), "encode_adjoint: unsupported N1");
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
+}
+
+int me_permute_shift(const void* x, const long long* x_strides, const int* x_dims, void* y,
+                     const long long* y_strides, const int* y_dims, const int* offset, float scale,
+                     hipStream_t stream) {
+  ME_REQUIRE(x && y && x_strides && y_strides && x_dims && y_dims && offset, "permute_shift: null argument");
+  PermArgs P{};
+  P.x = (const float2*)x;
+  P.y = (float2*)y;
+  P.scale = scale;
+  P.total = 1;
+  for (int a = 0; a < 4; ++a) {
+    ME_REQUIRE(y_dims[a] >= 1 && x_dims[a] >= y_dims[a], "permute_shift: output extent exceeds input extent");
+    P.xs[a] = x_strides[a];
+    P.ys[a] = y_strides[a];
+    P.ydims[a] = y_dims[a];
+    P.xdims[a] = x_dims[a];
+    P.off[a] = ((offset[a] % x_dims[a]) + x_dims[a]) % x_dims[a];
+    P.total *= y_dims[a];
+    P.ord[a] = a;
+  }
+  for (int i = 0; i < 4; ++i)  // axes by decreasing output stride (slowest first)
+    for (int j = i + 1; j < 4; ++j)
+      if (P.ys[P.ord[j]] > P.ys[P.ord[i]]) {
+        const int tmp = P.ord[i];
+        P.ord[i] = P.ord[j];
+        P.ord[j] = tmp;
+      }
+  const unsigned blocks = (unsigned)((P.total + 255) / 256);
+  hipLaunchKernelGGL(k_permute_shift, dim3(blocks), dim3(256), 0, stream, P);
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
+}
+
+int me_coil_ssq(const void* maps, int ncoils, long long vol, float eps, float thr, float* P, unsigned char* M,
+                hipStream_t stream) {
+  ME_REQUIRE(maps && ncoils >= 1 && vol >= 1, "coil_ssq: bad argument");
+  hipLaunchKernelGGL(k_coil_ssq, dim3((unsigned)((vol + 255) / 256)), dim3(256), 0, stream, (const float2*)maps,
+                     ncoils, vol, eps, thr, P, M);
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
+}
+
+int me_gather_hybrid(const void* ksp_h, int ncoils, const int* n, const int* samp_k0, const int* samp_k1,
+                     const int* shot_off, const int* shot_cnt, int nshots, long long total_samples, float scale,
+                     void* ymeas, hipStream_t stream) {
+  ME_REQUIRE(ksp_h && n && samp_k0 && samp_k1 && shot_off && shot_cnt && ymeas, "gather_hybrid: null argument");
+  const long long work = total_samples * ncoils;
+  if (work == 0) return 0;
+  hipLaunchKernelGGL(k_gather_hybrid, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, stream,
+                     (const float2*)ksp_h, ncoils, n[0], n[1], n[2], samp_k0, samp_k1, shot_off, shot_cnt, nshots,
+                     total_samples, scale, (float2*)ymeas);
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
+}
+
+int me_scatter_hybrid(const void* y, int ncoils, const int* n, const int* samp_k0, const int* samp_k1,
+                      const int* shot_off, const int* shot_cnt, int nshots, long long total_samples, void* ksp_h,
+                      hipStream_t stream) {
+  ME_REQUIRE(y && n && samp_k0 && samp_k1 && shot_off && shot_cnt && ksp_h, "scatter_hybrid: null argument");
+  const long long work = total_samples * ncoils;
+  if (work == 0) return 0;
+  hipLaunchKernelGGL(k_scatter_hybrid, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, stream, (const float2*)y,
+                     ncoils, n[0], n[1], n[2], samp_k0, samp_k1, shot_off, shot_cnt, nshots, total_samples,
+                     (float2*)ksp_h);
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
+}
+
+int me_max_abs(const void* x, long long n, unsigned int* out_bits, hipStream_t stream) {
+  ME_REQUIRE(x && out_bits, "max_abs: null argument");
+  hipLaunchKernelGGL(k_maxabs, dim3(1024), dim3(256), 0, stream, (const float2*)x, n, out_bits);
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

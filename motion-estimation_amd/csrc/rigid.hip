@@ -235,7 +235,7 @@ struct Bufs {
 };
 
 int run_passes(const std::vector<Pass>& passes, const VolGeom& g, Bufs b, int nbatch, const double* coef,
-               float2* sum_out, const unsigned char* mask, hipStream_t stream) {
+               float2* sum_out, const unsigned char* mask, bool accumulate, hipStream_t stream) {
   for (size_t i = 0; i < passes.size(); ++i) {
     const Pass& p = passes[i];
     LineArgs A{};
@@ -261,6 +261,7 @@ int run_passes(const std::vector<Pass>& passes, const VolGeom& g, Bufs b, int nb
       A.out_bs[0] = 0;
       A.mask = mask;
       A.mask_bs = 0;
+      A.accumulate = accumulate ? 1 : 0;
     }
     int rc = launch_linepass(g, p.axis, p.nin, p.nout, A, p.ph, stream);
     if (rc) return rc;
@@ -312,11 +313,12 @@ int me_rigid_transform(const void* x, long long x_bstride, void* y, long long y_
   const bool sum = flags & ME_FLAG_SUM_BATCH;
   ME_REQUIRE(!sum || work, "rigid_transform: batch sum needs a work buffer");
   ME_REQUIRE(!(flags & ME_FLAG_MASK) || mask, "rigid_transform: mask flag without mask");
+  ME_REQUIRE(!(flags & ME_FLAG_ACCUMULATE) || sum, "rigid_transform: accumulate needs the batch sum");
   if (int rc = launch_coef(theta, theta_stride, coef_ws, nbatch, stream)) return rc;
   std::vector<Pass> passes = mode == 1 ? build_adjoint() : build_apply(mode == 2);
   Bufs b{(const float2*)x, x_bstride, (float2*)(sum ? work : y), sum ? work_bstride : y_bstride, nullptr, 0};
   return run_passes(passes, g, b, nbatch, coef_ws, sum ? (float2*)y : nullptr,
-                    (flags & ME_FLAG_MASK) ? mask : nullptr, stream);
+                    (flags & ME_FLAG_MASK) ? mask : nullptr, (flags & ME_FLAG_ACCUMULATE) != 0, stream);
 }
 
 int me_rigid_deriv(const void* x, long long x_bstride, void* y, long long y_bstride, void* work,
@@ -330,7 +332,7 @@ int me_rigid_deriv(const void* x, long long x_bstride, void* y, long long y_bstr
   ME_REQUIRE(p_idx < 3 || work, "rigid_deriv: rotation derivatives need a work buffer");
   if (int rc = launch_coef(theta, theta_stride, coef_ws, nbatch, stream)) return rc;
   Bufs b{(const float2*)x, x_bstride, (float2*)y, y_bstride, (float2*)work, work_bstride};
-  return run_passes(build_deriv(p_idx), g, b, nbatch, coef_ws, nullptr, nullptr, stream);
+  return run_passes(build_deriv(p_idx), g, b, nbatch, coef_ws, nullptr, nullptr, false, stream);
 }
 
 int me_fft_axis(const void* x, long long x_bstride, void* y, long long y_bstride, int nbatch, const int* n,

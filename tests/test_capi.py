@@ -13,7 +13,7 @@ HEADER = os.path.join(ROOT, "include", "motion_estimation_amd.h")
 
 def _declared():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(me_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:[\w\*]+\s+)+\*?(me_\w+)\s*\(", txt, re.M)))
 
 
 def test_header_declares_entry_points():

@@ -1,0 +1,179 @@
+"""GPU parity of encoding, CG-SENSE, LM and JointRecon against the FP64 oracle.
+
+Tolerances (written here, from the build's target): relative L2 1e-4 on
+images, k-space and residuals; 1e-3 (degrees for rotations, voxels for
+translations) on estimated motion.  The problem is small enough for the dense
+oracle (32 x 24 x 16, 4 coils, 8 DISORDER shots) and uses seeded synthetic
+inputs; the data are motion-corrupted with known per-shot parameters plus
+noise."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import motion_oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-4
+MOTION_TOL = 1e-3
+
+SHAPE = (32, 24, 16)
+NC = 4
+
+
+def _rel(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    b = np.asarray(b)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _motion_err(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    d = np.abs(np.asarray(a) - np.asarray(b))
+    return max(d[:, :3].max(), np.degrees(d[:, 3:]).max())
+
+
+@pytest.fixture(scope="module")
+def prob_data(me):
+    sim = me.simulation
+    gt = sim.phantom(SHAPE, seed=0).numpy().astype(np.complex128)
+    mps = sim.coil_maps(SHAPE, NC, seed=1).numpy().astype(np.complex128)
+    A, order = sim.disorder_sampling(SHAPE, (4, 4), 2)
+    t_true = sim.motion_uniform(A.shape[0], rot_deg=3.0, trans_vox=1.5, seed=2)
+    ksp = O.generate_corrupted_kspace(gt, mps, A, t_true)
+    rng = np.random.default_rng(3)
+    ksp = ksp + 1e-3 * np.abs(ksp).max() * (rng.standard_normal(ksp.shape) + 1j * rng.standard_normal(ksp.shape))
+    # uncentred copies (the frame estimate_image_cg / estimate_transform expect)
+    ax = (-3, -2, -1)
+    ksp_u = np.fft.ifftshift(ksp, axes=ax)
+    mps_u = np.fft.ifftshift(mps, axes=ax)
+    A_u = np.fft.ifftshift(A, axes=(1, 2))
+    kg, rkg = me.compute_transform_grids_voxel(SHAPE, (1, 1, 1))
+    okg, orkg = O.compute_transform_grids_voxel(SHAPE, (1, 1, 1))
+    ssq = np.sum(np.abs(mps_u) ** 2, axis=0)
+    P = (1 / (ssq + 1e-3)).astype(np.float32)
+    M = ssq > 0.1
+    return dict(gt=gt, mps=mps, A=A, ksp=ksp, t_true=t_true, ksp_u=ksp_u, mps_u=mps_u, A_u=A_u, kg=kg, rkg=rkg,
+                okg=okg, orkg=orkg, P=P, M=M)
+
+
+def _problem(me, d):
+    return me.encoding.SenseProblem(d["ksp_u"], d["mps_u"], d["A_u"], d["kg"], d["rkg"])
+
+
+def test_corrupted_kspace_matches_oracle(me, prob_data):
+    d = prob_data
+    got = me.simulation.generate_corrupted_kspace(d["gt"], d["mps"], d["A"], d["t_true"])
+    ref = O.generate_corrupted_kspace(d["gt"], d["mps"], d["A"], d["t_true"])
+    assert _rel(got, ref) < TOL
+
+
+def test_forward_encoding_matches_oracle(me, prob_data):
+    d = prob_data
+    pb = _problem(me, d)
+    pb.theta.copy_(torch.as_tensor(d["t_true"]))
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(SHAPE) + 1j * rng.standard_normal(SHAPE)
+    Y = pb.forward(pb.image_to_plane(x)).cpu().numpy()
+    pl = pb.plan
+    n2 = SHAPE[2]
+    for s in range(pb.nshots):
+        dense = O.encode_forward(x, d["mps_u"], d["A_u"][s], d["t_true"][s], d["okg"], d["orkg"])
+        hyb = np.fft.ifft(dense, axis=-1, norm="ortho")  # readout inverse FFT -> hybrid space
+        o, c = pl.off_h[s], pl.cnt_h[s]
+        ref = hyb[:, pl.k0_h[o:o + c], pl.k1_h[o:o + c], :]  # (C, cnt, n2)
+        got = Y[o * n2 * NC:(o + c) * n2 * NC].reshape(n2, NC, c)
+        assert _rel(np.transpose(got, (1, 2, 0)), ref) < TOL
+        # nothing outside the mask
+        assert np.abs(dense[:, ~d["A_u"][s][..., 0]]).max() == 0
+
+
+def test_encoding_adjoint_identity(me, prob_data):
+    d = prob_data
+    pb = _problem(me, d)
+    rng = np.random.default_rng(5)
+    x = pb.image_to_plane(rng.standard_normal(SHAPE) + 1j * rng.standard_normal(SHAPE))
+    U = torch.stack([x] * pb.nshots)
+    Y = torch.empty_like(pb.ymeas)
+    pb._encode(U, pb.vol, 0, pb.nshots, y=Y)
+    y2 = torch.randn_like(Y)
+    W = torch.empty_like(U)
+    pb._decode(y2, 0, pb.nshots, W)
+    lhs = torch.vdot(Y, y2).cpu().numpy()
+    rhs = sum(torch.vdot(U[s], W[s]).cpu().numpy() for s in range(pb.nshots))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+def test_normal_op_and_rhs_match_oracle(me, prob_data):
+    d = prob_data
+    pb = _problem(me, d)
+    pb.theta.copy_(torch.as_tensor(d["t_true"]))
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(SHAPE) + 1j * rng.standard_normal(SHAPE)
+    out = pb.zeros_image()
+    pb.normal(pb.image_to_plane(x), out)
+    ref = O.normal_op(x, d["mps_u"], d["A_u"], d["t_true"], d["okg"], d["orkg"], d["M"])
+    assert _rel(pb.image_from_plane(out), ref) < TOL
+    b = pb.rhs(cache=False)
+    refb = O.cg_rhs(d["ksp_u"], d["mps_u"], d["A_u"], d["t_true"], d["okg"], d["orkg"], d["M"])
+    assert _rel(pb.image_from_plane(b), refb) < TOL
+    b2 = pb.rhs(cache=True)
+    assert _rel(pb.image_from_plane(b2), refb) < TOL
+
+
+def test_objective_matches_oracle(me, prob_data):
+    d = prob_data
+    pb = _problem(me, d)
+    pb.theta.copy_(torch.as_tensor(d["t_true"]))
+    x = d["gt"] * 0.9
+    got = pb.objective(pb.image_to_plane(np.fft.ifftshift(x)))
+    ref = sum(np.linalg.norm(O.shot_residual(d["ksp_u"], d["mps_u"], d["A_u"][s], np.fft.ifftshift(x),
+                                             d["t_true"][s], d["okg"], d["orkg"])) ** 2 for s in range(8))
+    assert abs(got - ref) / ref < TOL
+
+
+@pytest.mark.parametrize("max_iter,tol", [(5, 1e-6), (20, 1e-2)])
+def test_estimate_image_cg_matches_oracle(me, prob_data, max_iter, tol):
+    d = prob_data
+    args = (d["ksp_u"], d["mps_u"], d["A_u"], d["t_true"])
+    got = me.estimate_image_cg(*args, d["kg"], d["rkg"], d["P"], d["M"], max_iter=max_iter, tol=tol)
+    ref = O.estimate_image_cg(*args, d["okg"], d["orkg"], d["P"], d["M"], max_iter=max_iter, tol=tol)
+    assert _rel(got, ref) < TOL
+
+
+def test_estimate_transform_matches_oracle(me, prob_data):
+    d = prob_data
+    x = O.estimate_image_cg(d["ksp_u"], d["mps_u"], d["A_u"], d["t_true"], d["okg"], d["orkg"], d["P"], d["M"],
+                            max_iter=5)
+    t0 = d["t_true"] + np.random.default_rng(7).normal(0, 0.05, d["t_true"].shape)
+    ns = t0.shape[0]
+    t_ref, damp_ref, fl_ref = t0.copy(), np.ones(ns), np.zeros(ns, dtype=bool)
+    trace = []
+    x_ref = O.estimate_transform(d["ksp_u"], d["mps_u"], d["A_u"], x, d["okg"], d["orkg"], damp_ref, fl_ref, t_ref,
+                                 trace=trace)
+    t_got, damp_got, fl_got = t0.copy(), np.ones(ns), np.zeros(ns, dtype=bool)
+    x_got = me.estimate_transform(d["ksp_u"], d["mps_u"], d["A_u"], x, d["kg"], d["rkg"], damp_got, fl_got, None, None,
+                                  None, None, transforms=t_got)
+    assert np.array_equal(damp_got, damp_ref), (damp_got, damp_ref, [tr[:3] for tr in trace])
+    assert _motion_err(t_got, t_ref) < MOTION_TOL
+    assert _rel(x_got, x_ref) < TOL
+
+
+def test_estimate_transform_requires_transforms(me, prob_data):
+    d = prob_data
+    with pytest.raises(TypeError):
+        me.estimate_transform(d["ksp_u"], d["mps_u"], d["A_u"], d["gt"], d["kg"], d["rkg"], np.ones(8),
+                              np.zeros(8, bool), None, None, None, None, transforms=None)
+
+
+def test_joint_recon_matches_oracle(me, prob_data):
+    d = prob_data
+    kw = dict(max_cg_iter=3, max_joint_iter=3)
+    app = me.JointRecon(d["ksp"], d["mps"], d["A"], d["kg"], d["rkg"], **kw)
+    x_got, t_got = app.run()
+    ref = O.JointRecon(d["ksp"], d["mps"], d["A"], d["okg"], d["orkg"], **kw)
+    x_ref, t_ref = ref.run()
+    assert abs(app.max_norm - ref.max_norm) / ref.max_norm < 1e-5
+    assert _rel(x_got, x_ref) < TOL
+    assert _motion_err(t_got, t_ref) < MOTION_TOL
+    assert np.allclose(app.objective_history, ref.objective_history, rtol=TOL, atol=0)
+    assert app.transform_history.shape == ref.transform_history.shape
