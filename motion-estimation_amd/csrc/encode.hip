@@ -130,14 +130,30 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
   auto lidx = [&](int k1, int m2) { return g * (R1 * (R2 + 1)) + k1 * (R2 + 1) + m2; };
   __syncthreads();
 
+  // rows are streamed with the next row's map/image values prefetched into
+  // registers while the current row is transformed (loads are unconditional:
+  // invalid coils read coil 0 and are zeroed by the mask factor)
+  const float cm = cvalid ? 1.f : 0.f;
+  float2 sv[R1], uv[R1];
+  if (t < R2) {
+#pragma unroll
+    for (int m1 = 0; m1 < R1; ++m1) {
+      sv[m1] = sp[R2 * m1 + t];
+      uv[m1] = up[R2 * m1 + t];
+    }
+  }
   for (int x0 = 0; x0 < n0; ++x0) {
     if (t < R2) {
       float2 a[R1];
-      const int ro = x0 * n1;
 #pragma unroll
-      for (int m1 = 0; m1 < R1; ++m1) {
-        const int pos = ro + R2 * m1 + t;
-        a[m1] = cvalid ? c_mul(sp[pos], up[pos]) : make_float2(0.f, 0.f);
+      for (int m1 = 0; m1 < R1; ++m1) a[m1] = c_scale(c_mul(sv[m1], uv[m1]), cm);
+      if (x0 + 1 < n0) {
+        const int ro = (x0 + 1) * n1 + t;
+#pragma unroll
+        for (int m1 = 0; m1 < R1; ++m1) {
+          sv[m1] = sp[ro + R2 * m1];
+          uv[m1] = up[ro + R2 * m1];
+        }
       }
       RDFT<R1, -1>::run(a);
 #pragma unroll
@@ -236,17 +252,33 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
     for (int s0 = 0; s0 < cnt; s0 += chunkmax) {
       const int s1 = min(cnt, s0 + chunkmax);
       __syncthreads();  // previous users of dyn/xch are done
+      const int nchunk_s = s1 - s0;
+      // stage the coil group's samples: loads issued 8 at a time from clamped
+      // addresses, zero for absent coils
+#pragma unroll 8
       for (int j = tid; j < G * chunkmax; j += NT) {
-        const int q = j / chunkmax, w = s0 + j % chunkmax;
+        const int q = j / chunkmax, w = j % chunkmax;
         const int cc = cg * G + q;
-        dyn[j] = (cc < A.ncoils && w < s1) ? A.y[ybase + ((long long)x2 * A.ncoils + cc) * cnt + w]
-                                            : make_float2(0.f, 0.f);
+        const bool ok = cc < A.ncoils && w < nchunk_s;
+        const long long src = ybase + ((long long)x2 * A.ncoils + (ok ? cc : 0)) * cnt + s0 + (ok ? w : 0);
+        const float2 v = A.y[src];
+        dyn[j] = ok ? v : make_float2(0.f, 0.f);
       }
-      for (int j = tid; j < s1 - s0; j += NT) sk0l[j] = A.sk0[off + s0 + j];
+      for (int j = tid; j < nchunk_s; j += NT) sk0l[j] = A.sk0[off + s0 + j];
       __syncthreads();
+      const bool add_prev = cg > 0 || s0 > 0;
+      const float cm = cvalid ? 1.f : 0.f;
       for (int r = 0; r < RB; ++r) {
         const int x0 = row0 + r;
         if (x0 >= n0) break;
+        // loads used late in the row are issued first
+        float2 sv[R2];
+        if (t < R1) {
+#pragma unroll
+          for (int k2 = 0; k2 < R2; ++k2) sv[k2] = sp[x0 * n1 + t + R1 * k2];
+        }
+        float2 prev = make_float2(0.f, 0.f);
+        if (add_prev && tid < NR) prev = op[x0 * n1 + tid];
         if (t < R2) {
           float2 a[R1];
 #pragma unroll
@@ -277,19 +309,14 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
         // xch reused as the per-coil product buffer [G][NR]
         if (t < R1) {
 #pragma unroll
-          for (int k2 = 0; k2 < R2; ++k2) {
-            const int pos = t + R1 * k2;
-            xch[g * NR + pos] = cvalid ? c_mulc(X[k2], sp[x0 * n1 + pos]) : make_float2(0.f, 0.f);
-          }
+          for (int k2 = 0; k2 < R2; ++k2) xch[g * NR + t + R1 * k2] = c_scale(c_mulc(X[k2], sv[k2]), cm);
         }
         __syncthreads();
         if (tid < NR) {  // coil sum; the first contribution writes, later ones add
           float2 s = make_float2(0.f, 0.f);
 #pragma unroll
           for (int q = 0; q < G; ++q) s = c_add(s, xch[q * NR + tid]);
-          s = c_scale(s, scale);
-          if (cg > 0 || s0 > 0) s = c_add(s, op[x0 * n1 + tid]);
-          op[x0 * n1 + tid] = s;
+          op[x0 * n1 + tid] = c_add(c_scale(s, scale), prev);
         }
         __syncthreads();
       }

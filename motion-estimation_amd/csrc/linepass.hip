@@ -31,14 +31,16 @@ struct LineT {
   static constexpr int NT = T * kLinesPerWG;
 };
 
-// Phase vector for one (output, input) pair on the k values owned by a
-// step-2 thread: k = t + R1*k2, k2 = 0..R2-1.
+// acc[k2] += phi(k) * X[k2] for one (output, input) pair on the k values owned
+// by a step-2 thread, k = t + R1*k2.  The phase exp(i k_a beta) is factored as
+// exp(2 pi i beta t/N) * exp(2 pi i beta R1/N)^k2 * (k >= N/2 ? exp(-2 pi i beta) : 1)
+// with each factor reduced in FP64 before the FP32 sincos.
 template <int N, int R1, int R2>
-__device__ __forceinline__ void phase_vec(const PhaseSpec& P, const double* cf, double rb, int t,
-                                          float2 (&ph)[R2]) {
+__device__ __forceinline__ void phase_acc(const PhaseSpec& P, const double* cf, double rb, int t,
+                                          const float2 (&X)[R2], float2 (&acc)[R2]) {
   const double c = P.ic >= 0 ? (double)P.sc * cf[P.ic] : 0.0;
   const double d = P.id >= 0 ? (double)P.sd * cf[P.id] : 0.0;
-  const double beta = c * rb + d;  // shift (in samples) scaled so that angle = k_a * beta
+  const double beta = c * rb + d;  // angle = k_a * beta
   float2 cur = cis2pi(frac_half(beta * (double)t / (double)N));
   const float2 step = cis2pi(frac_half(beta * (double)R1 / (double)N));
   const float2 wrap = cis2pi(frac_half(-beta));
@@ -56,12 +58,15 @@ __device__ __forceinline__ void phase_vec(const PhaseSpec& P, const double* cf, 
       const float kap = (float)(k - (w ? N : 0)) / (float)N;
       e = c_muli(c_scale(e, gam * kap));
     }
-    ph[k2] = e;
-    cur = c_mul(cur, step);
+    acc[k2] = c_fma(X[k2], e, acc[k2]);
+    if (k2 + 1 < R2) cur = c_mul(cur, step);
   }
 }
 
-template <int N, bool CONTIG, int NIN, int NOUT>
+// MODE 0: FFT, phase, inverse FFT; 1: forward FFT only (natural-order
+// spectrum out); 2: inverse FFT only (natural-order spectrum in).
+// REDUCE: sum the result over the batch and write batch entry 0 only.
+template <int N, bool CONTIG, int NIN, int NOUT, int MODE, bool REDUCE>
 __global__ void __launch_bounds__(LineT<N>::NT) k_linepass(LineArgs A) {
   constexpr int R1 = LineT<N>::R1;
   constexpr int R2 = LineT<N>::R2;
@@ -95,14 +100,17 @@ __global__ void __launch_bounds__(LineT<N>::NT) k_linepass(LineArgs A) {
   };
   const double ru = A.r_u ? A.r_u[u] : 0.0;
   const double rv = A.r_v ? A.r_v[v] : 0.0;
+  // padding lines (valid == false) read line 0 and are zeroed, so every load
+  // is unconditional and a thread's loads are all in flight together
+  const float vm = valid ? 1.f : 0.f;
+  const float inv_scale = MODE == 0 ? 1.0f / (float)N : rsqrtf((float)N);
   __syncthreads();
 
-  const float inv_scale = A.mode == 0 ? 1.0f / (float)N : rsqrtf((float)N);
-  const int b_begin = A.reduce ? 0 : (int)blockIdx.y;
-  const int b_end = A.reduce ? A.nbatch : (int)blockIdx.y + 1;
-  float2 red[R1];
+  const int b_begin = REDUCE ? 0 : (int)blockIdx.y;
+  const int b_end = REDUCE ? A.nbatch : (int)blockIdx.y + 1;
+  float2 red[REDUCE ? R1 : 1];
 #pragma unroll
-  for (int i = 0; i < R1; ++i) red[i] = make_float2(0.f, 0.f);
+  for (int i = 0; i < (REDUCE ? R1 : 1); ++i) red[i] = make_float2(0.f, 0.f);
 
   for (int b = b_begin; b < b_end; ++b) {
     const double* cf = A.coef ? A.coef + (long long)b * A.ncoef : nullptr;
@@ -111,19 +119,25 @@ __global__ void __launch_bounds__(LineT<N>::NT) k_linepass(LineArgs A) {
 #pragma unroll
     for (int i = 0; i < NIN; ++i) {
       const float2* src = A.in[i] + (long long)b * A.in_bs[i] + line_off;
-      if (A.mode == 2) {  // input already in the frequency domain, natural order
+      if (MODE == 2) {
         if (t < R1) {
+          const float2* st = src + (long long)t * A.st_a;
+          const long long step = (long long)R1 * A.st_a;
 #pragma unroll
-          for (int k2 = 0; k2 < R2; ++k2)
-            X[i][k2] = valid ? src[(long long)(t + R1 * k2) * A.st_a] : make_float2(0.f, 0.f);
+          for (int k2 = 0; k2 < R2; ++k2) X[i][k2] = st[k2 * step];
+#pragma unroll
+          for (int k2 = 0; k2 < R2; ++k2) X[i][k2] = c_scale(X[i][k2], vm);
         }
         continue;
       }
       if (t < R2) {
         float2 a[R1];
+        const float2* st = src + (long long)t * A.st_a;
+        const long long step = (long long)R2 * A.st_a;
 #pragma unroll
-        for (int n1 = 0; n1 < R1; ++n1)
-          a[n1] = valid ? src[(long long)(R2 * n1 + t) * A.st_a] : make_float2(0.f, 0.f);
+        for (int n1 = 0; n1 < R1; ++n1) a[n1] = st[n1 * step];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) a[n1] = c_scale(a[n1], vm);
         RDFT<R1, -1>::run(a);
 #pragma unroll
         for (int k1 = 1; k1 < R1; ++k1) a[k1] = c_mul(a[k1], tw[(t * k1) % N]);
@@ -154,23 +168,21 @@ __global__ void __launch_bounds__(LineT<N>::NT) k_linepass(LineArgs A) {
 #pragma unroll
             for (int k2 = 0; k2 < R2; ++k2) acc[k2] = c_add(acc[k2], X[i][k2]);
           } else {
-            float2 ph[R2];
             const double rb = P.slot == 0 ? ru : (P.slot == 1 ? rv : 0.0);
-            phase_vec<N, R1, R2>(P, cf, rb, t, ph);
-#pragma unroll
-            for (int k2 = 0; k2 < R2; ++k2) acc[k2] = c_fma(X[i][k2], ph[k2], acc[k2]);
+            phase_acc<N, R1, R2>(P, cf, rb, t, X[i], acc);
           }
         }
       }
-      float2* dst = A.out[o] + (long long)(A.reduce ? 0 : b) * A.out_bs[o] + line_off;
-      if (A.mode == 1) {  // forward only: store natural-order spectrum
+      float2* dst = A.out[o] + (long long)(REDUCE ? 0 : b) * A.out_bs[o] + line_off;
+      if (MODE == 1) {  // forward only: store natural-order spectrum
         if (t < R1 && valid) {
+          float2* dt = dst + (long long)t * A.st_a;
+          const long long step = (long long)R1 * A.st_a;
 #pragma unroll
           for (int k2 = 0; k2 < R2; ++k2) {
-            const long long pos = (long long)(t + R1 * k2) * A.st_a;
             float2 val = c_scale(acc[k2], inv_scale);
-            if (A.accumulate) val = c_add(val, dst[pos]);
-            dst[pos] = val;
+            if (A.accumulate) val = c_add(val, dt[k2 * step]);
+            dt[k2 * step] = val;
           }
         }
         continue;
@@ -188,60 +200,67 @@ __global__ void __launch_bounds__(LineT<N>::NT) k_linepass(LineArgs A) {
 #pragma unroll
         for (int k1 = 0; k1 < R1; ++k1) a[k1] = lds[lidx(k1, t)];
         RDFT<R1, +1>::run(a);
-        if (A.reduce) {
+        if (REDUCE) {
 #pragma unroll
-          for (int n1 = 0; n1 < R1; ++n1) red[n1] = c_add(red[n1], a[n1]);
+          for (int n1 = 0; n1 < R1; ++n1) red[n1 < (REDUCE ? R1 : 1) ? n1 : 0] = c_add(red[n1 < (REDUCE ? R1 : 1) ? n1 : 0], a[n1]);
         } else if (valid) {
+          float2* dt = dst + (long long)t * A.st_a;
+          const long long step = (long long)R2 * A.st_a;
+          const unsigned char* mk = A.mask ? A.mask + (long long)b * A.mask_bs + line_off + (long long)t * A.st_a : nullptr;
 #pragma unroll
           for (int n1 = 0; n1 < R1; ++n1) {
-            const long long pos = (long long)(R2 * n1 + t) * A.st_a;
             float2 val = c_scale(a[n1], inv_scale);
-            if (A.mask && !A.mask[(long long)b * A.mask_bs + line_off + pos]) val = make_float2(0.f, 0.f);
-            if (A.accumulate) val = c_add(val, dst[pos]);
-            dst[pos] = val;
+            if (mk && !mk[n1 * step]) val = make_float2(0.f, 0.f);
+            if (A.accumulate) val = c_add(val, dt[n1 * step]);
+            dt[n1 * step] = val;
           }
         }
       }
       __syncthreads();
     }
   }
-  if (A.reduce && t < R2 && valid) {
-    float2* dst = A.out[0] + line_off;
+  if (REDUCE && t < R2 && valid) {
+    float2* dt = A.out[0] + line_off + (long long)t * A.st_a;
+    const long long step = (long long)R2 * A.st_a;
+    const unsigned char* mk = A.mask ? A.mask + line_off + (long long)t * A.st_a : nullptr;
 #pragma unroll
     for (int n1 = 0; n1 < R1; ++n1) {
-      const long long pos = (long long)(R2 * n1 + t) * A.st_a;
-      float2 val = c_scale(red[n1], inv_scale);
-      if (A.mask && !A.mask[line_off + pos]) val = make_float2(0.f, 0.f);
-      if (A.accumulate) val = c_add(val, dst[pos]);
-      dst[pos] = val;
+      float2 val = c_scale(red[n1 < (REDUCE ? R1 : 1) ? n1 : 0], inv_scale);
+      if (mk && !mk[n1 * step]) val = make_float2(0.f, 0.f);
+      if (A.accumulate) val = c_add(val, dt[n1 * step]);
+      dt[n1 * step] = val;
     }
   }
 }
 
 namespace {
-template <bool CONTIG, int NIN, int NOUT>
+template <bool CONTIG, int NIN, int NOUT, int MODE, bool REDUCE>
 struct LineLaunch {
   const LineArgs* args;
   dim3 grid;
   hipStream_t stream;
   template <int N>
   void operator()() {
-    hipLaunchKernelGGL((k_linepass<N, CONTIG, NIN, NOUT>), grid, dim3(LineT<N>::NT), 0, stream, *args);
+    hipLaunchKernelGGL((k_linepass<N, CONTIG, NIN, NOUT, MODE, REDUCE>), grid, dim3(LineT<N>::NT), 0, stream,
+                       *args);
   }
 };
 
-template <bool CONTIG, int NIN, int NOUT>
+template <bool CONTIG, int NIN, int NOUT, int MODE, bool REDUCE>
 bool launch_n(int n, const LineArgs& a, dim3 grid, hipStream_t s) {
-  LineLaunch<CONTIG, NIN, NOUT> f{&a, grid, s};
+  LineLaunch<CONTIG, NIN, NOUT, MODE, REDUCE> f{&a, grid, s};
   return dispatch_line_length(n, f);
 }
 
 template <bool CONTIG>
 bool launch_io(int n, int nin, int nout, const LineArgs& a, dim3 grid, hipStream_t s) {
-  if (nin == 1 && nout == 1) return launch_n<CONTIG, 1, 1>(n, a, grid, s);
-  if (nin == 1 && nout == 2) return launch_n<CONTIG, 1, 2>(n, a, grid, s);
-  if (nin == 2 && nout == 2) return launch_n<CONTIG, 2, 2>(n, a, grid, s);
-  if (nin == 2 && nout == 1) return launch_n<CONTIG, 2, 1>(n, a, grid, s);
+  if (a.mode == 1) return nin == 1 && nout == 1 && launch_n<CONTIG, 1, 1, 1, false>(n, a, grid, s);
+  if (a.mode == 2) return nin == 1 && nout == 1 && launch_n<CONTIG, 1, 1, 2, false>(n, a, grid, s);
+  if (a.reduce) return nin == 1 && nout == 1 && launch_n<CONTIG, 1, 1, 0, true>(n, a, grid, s);
+  if (nin == 1 && nout == 1) return launch_n<CONTIG, 1, 1, 0, false>(n, a, grid, s);
+  if (nin == 1 && nout == 2) return launch_n<CONTIG, 1, 2, 0, false>(n, a, grid, s);
+  if (nin == 2 && nout == 2) return launch_n<CONTIG, 2, 2, 0, false>(n, a, grid, s);
+  if (nin == 2 && nout == 1) return launch_n<CONTIG, 2, 1, 0, false>(n, a, grid, s);
   return false;
 }
 }  // namespace
@@ -250,7 +269,8 @@ int launch_linepass(const VolGeom& g, int axis, int nin, int nout, LineArgs A, c
                     hipStream_t stream) {
   ME_REQUIRE(axis >= 0 && axis < 3, "line pass: axis out of range");
   ME_REQUIRE(nin >= 1 && nin <= 2 && nout >= 1 && nout <= 2, "line pass: 1 or 2 inputs/outputs");
-  ME_REQUIRE(!(A.reduce && nout != 1), "line pass: batch reduction needs a single output");
+  ME_REQUIRE(!(A.reduce && (nout != 1 || nin != 1)), "line pass: batch reduction needs one input and output");
+  ME_REQUIRE(A.mode == 0 || (nin == 1 && nout == 1 && !A.reduce), "line pass: FFT-only modes are single-buffer");
   ME_REQUIRE(A.nbatch >= 1, "line pass: empty batch");
   const int o1 = axis == 0 ? 1 : 0;
   const int o2 = axis == 2 ? 1 : 2;
