@@ -49,9 +49,11 @@ int me_version(void);
  * flags ME_FLAG_SUM_BATCH (mode 1): y = sum_b T_b^H x_b, batch intermediates in
  *   `work` (work_bstride); with ME_FLAG_MASK, y is multiplied by mask (uint8,
  *   same layout as y).  This is the shot sum of image_estimation.py:128-129.
+ * x_parts (1 or 2): input b is the sum of x_parts volumes at
+ *   x + b*x_bstride + p*N0*N1*N2 (the coil-group partials of me_encode_adjoint).
  * theta: nbatch rows at theta_stride doubles; coef_ws: device scratch of
  *   16*nbatch doubles.  x and y may alias (in place) unless SUM_BATCH. */
-int me_rigid_transform(const void* x, long long x_bstride, void* y, long long y_bstride, void* work,
+int me_rigid_transform(const void* x, long long x_bstride, int x_parts, void* y, long long y_bstride, void* work,
                        long long work_bstride, int nbatch, const int* n, const long long* st, const double* r0,
                        const double* r1, const double* r2, const double* theta, long long theta_stride,
                        double* coef_ws, int mode, int flags, const unsigned char* mask, hipStream_t stream);
@@ -97,9 +99,15 @@ int me_encode_forward(const void* u, long long u_bstride, const void* maps, int 
                       const int* samp_k0, const int* samp_k1, const int* shot_off, const int* shot_cnt, int max_cnt,
                       void* y, const void* y_meas, double* partials, double* norms, hipStream_t stream);
 
-/* Adjoint of me_encode_forward for a batch of shots:
- *   out_s = sum_c conj(S_c) F_01^H (M_s Y_s)     (image_estimation.py:94-102,
- *   before T^H).  out at out_bstride per shot (plane-major). */
+/* Number of coil groups (16 coils each) the adjoint splits the coils into. */
+int me_encode_coil_groups(int ncoils);
+
+/* Adjoint of me_encode_forward for a batch of shots, per coil group g:
+ *   part_{s,g} = sum_{c in g} conj(S_c) F_01^H (M_s Y_{s,c})   (image_estimation.py:94-102,
+ *   before T^H); sum_g part_{s,g} is E_s^H Y_s.  Part g of shot s is written at
+ *   out + (s*ngroups + g)*out_bstride (plane-major); with more than two groups,
+ *   groups 2.. are folded into parts 0 and 1, so the sum of parts 0 and 1 is the
+ *   result (me_rigid_transform's x_parts = min(ngroups, 2) consumes them). */
 int me_encode_adjoint(const void* y, const void* maps, int ncoils, int nbatch, const int* n, const int* samp_k0,
                       const int* shot_off, const int* shot_cnt, const int* colptr, void* out, long long out_bstride,
                       hipStream_t stream);

@@ -26,7 +26,7 @@ dp = C.c_void_p  # device pointers are passed as void*
 # name -> argtypes (restype int unless noted)
 _SIGS = {
     "me_version": [],
-    "me_rigid_transform": [vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp, i32, i32, dp, vp],
+    "me_rigid_transform": [vp, i64, i32, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp, i32, i32, dp, vp],
     "me_rigid_deriv": [vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp, i32, vp],
     "me_fft_axis": [vp, i64, vp, i64, i32, vp, vp, i32, i32, vp],
     "me_encode_forward": [vp, i64, vp, i32, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp],
@@ -51,6 +51,7 @@ _SIZES = {
     "me_encode_partials_size": ([i32, i32, i32, i32], C.c_longlong),
     "me_vec_partials_size": ([], C.c_int),
     "me_lm_partials_size": ([i32], C.c_longlong),
+    "me_encode_coil_groups": ([i32], C.c_int),
 }
 _OPTIONAL = {}
 

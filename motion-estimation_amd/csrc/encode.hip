@@ -205,13 +205,20 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
   }
 }
 
+// Adjoint.  One workgroup per (readout plane x2, coil group, shot): the
+// group's samples are staged in LDS once, then all N0 rows are streamed:
+//   Z_c(x0, k1) = sum_{k0 in column k1} e^{+2 pi i k0 x0/N0} Y_c(k0, k1)   (scatter)
+//   z_c(x0, :)  = IFFT_1 Z_c(x0, :)                                         (row IFFT)
+//   part(x0, :) = sum_{c in group} conj(S_c(x0, :)) z_c(x0, :)              (coil sum)
+// Each coil group writes its own partial image (part cg of the shot); the
+// adjoint rigid transform's first pass sums the parts.
 template <int NR>
 __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
   constexpr int R1 = RowT<NR>::R1, R2 = RowT<NR>::R2, T = RowT<NR>::T, NT = RowT<NR>::NT;
   constexpr int G = kCoilsPerWG;
-  constexpr int RB = 16;  // rows per workgroup (output accumulators per thread)
-  extern __shared__ float2 dyn[];  // [G][chunk] samples of the coil group
+  extern __shared__ float2 dyn[];  // [G][kAdjChunk] samples of the coil group
   __shared__ float2 xch[G * R1 * (R2 + 1)];
+  __shared__ float2 prod[G * NR];
   __shared__ float2 twr[NR];
   __shared__ float2 twc[kMaxCol];
   __shared__ int cp[NR + 1];
@@ -219,14 +226,14 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
 
   const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
   const int b = blockIdx.y;
-  const int nrb = (n0 + RB - 1) / RB;
   const int x2 = blockIdx.x % n2;
-  const int rb = blockIdx.x / n2;
-  const int row0 = rb * RB;
+  const int cg = blockIdx.x / n2;
   const int tid = threadIdx.x;
   const int t = tid % T, g = tid / T;
   const int cnt = A.cnt[b], off = A.off[b];
-  const int chunkmax = kAdjChunk;  // LDS capacity in samples per coil
+  const int c = cg * G + g;
+  const bool cvalid = c < A.ncoils;
+  const float cm = cvalid ? 1.f : 0.f;
 
   fill_twiddles<NR, NT>(twr);
   for (int j = tid; j < n0; j += NT) {
@@ -237,97 +244,97 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
   }
   for (int j = tid; j <= n1; j += NT) cp[j] = A.colptr[(long long)b * (n1 + 1) + j];
 
-  const float scale = rsqrtf((float)n0 * (float)n1);
-  float2* op = A.out + (long long)b * A.out_bs + (long long)x2 * (long long)n0 * n1;
   const long long plane = (long long)n0 * n1;
   const long long vol = plane * n2;
   const long long ybase = (long long)off * n2 * A.ncoils;
+  const float scale = rsqrtf((float)n0 * (float)n1);
+  const float2* sp = A.maps + (long long)(cvalid ? c : 0) * vol + (long long)x2 * plane;
+  float2* op = A.out + ((long long)b * A.ncg + cg) * A.out_bs + (long long)x2 * plane;
   auto lidx = [&](int k1, int m2) { return g * (R1 * (R2 + 1)) + k1 * (R2 + 1) + m2; };
-  (void)nrb;
 
-  for (int cg = 0; cg < A.ncg; ++cg) {
-    const int c = cg * G + g;
-    const bool cvalid = c < A.ncoils;
-    const float2* sp = A.maps + (long long)(cvalid ? c : 0) * vol + (long long)x2 * plane;
-    for (int s0 = 0; s0 < cnt; s0 += chunkmax) {
-      const int s1 = min(cnt, s0 + chunkmax);
-      __syncthreads();  // previous users of dyn/xch are done
-      const int nchunk_s = s1 - s0;
-      // stage the coil group's samples: loads issued 8 at a time from clamped
-      // addresses, zero for absent coils
+  if (cnt == 0) {  // a shot without samples contributes zero
+    for (long long j = tid; j < plane; j += NT) op[j] = make_float2(0.f, 0.f);
+    return;
+  }
+  for (int s0 = 0; s0 < cnt; s0 += kAdjChunk) {
+    const int ns = min(kAdjChunk, cnt - s0);
+    __syncthreads();  // previous chunk's readers are done
 #pragma unroll 8
-      for (int j = tid; j < G * chunkmax; j += NT) {
-        const int q = j / chunkmax, w = j % chunkmax;
-        const int cc = cg * G + q;
-        const bool ok = cc < A.ncoils && w < nchunk_s;
-        const long long src = ybase + ((long long)x2 * A.ncoils + (ok ? cc : 0)) * cnt + s0 + (ok ? w : 0);
-        const float2 v = A.y[src];
-        dyn[j] = ok ? v : make_float2(0.f, 0.f);
+    for (int j = tid; j < G * kAdjChunk; j += NT) {
+      const int q = j / kAdjChunk, w = j % kAdjChunk;
+      const int cc = cg * G + q;
+      const bool ok = cc < A.ncoils && w < ns;
+      const long long src = ybase + ((long long)x2 * A.ncoils + (ok ? cc : 0)) * cnt + s0 + (ok ? w : 0);
+      const float2 v = A.y[src];
+      dyn[j] = ok ? v : make_float2(0.f, 0.f);
+    }
+    for (int j = tid; j < ns; j += NT) sk0l[j] = A.sk0[off + s0 + j];
+    __syncthreads();
+    const bool add_prev = s0 > 0;
+    float2 sv[R2];
+    if (t < R1) {
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) sv[k2] = sp[t + R1 * k2];
+    }
+    for (int x0 = 0; x0 < n0; ++x0) {
+      // step 1: scatter this row's spectrum, length-R1 inverse DFTs
+      if (t < R2) {
+        float2 a[R1];
+#pragma unroll
+        for (int m1 = 0; m1 < R1; ++m1) {
+          const int k1 = R2 * m1 + t;
+          float2 z = make_float2(0.f, 0.f);
+          const int w0 = max(cp[k1], s0) - s0, w1 = min(cp[k1 + 1], s0 + ns) - s0;
+          for (int w = w0; w < w1; ++w) z = c_fmac(twc[(sk0l[w] * x0) % n0], dyn[g * kAdjChunk + w], z);
+          a[m1] = z;
+        }
+        RDFT<R1, +1>::run(a);
+#pragma unroll
+        for (int k1 = 1; k1 < R1; ++k1) a[k1] = c_mulc(a[k1], twr[(t * k1) % NR]);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) xch[lidx(k1, t)] = a[k1];
       }
-      for (int j = tid; j < nchunk_s; j += NT) sk0l[j] = A.sk0[off + s0 + j];
       __syncthreads();
-      const bool add_prev = cg > 0 || s0 > 0;
-      const float cm = cvalid ? 1.f : 0.f;
-      for (int r = 0; r < RB; ++r) {
-        const int x0 = row0 + r;
-        if (x0 >= n0) break;
-        // loads used late in the row are issued first
-        float2 sv[R2];
-        if (t < R1) {
-#pragma unroll
-          for (int k2 = 0; k2 < R2; ++k2) sv[k2] = sp[x0 * n1 + t + R1 * k2];
-        }
-        float2 prev = make_float2(0.f, 0.f);
-        if (add_prev && tid < NR) prev = op[x0 * n1 + tid];
-        if (t < R2) {
-          float2 a[R1];
-#pragma unroll
-          for (int m1 = 0; m1 < R1; ++m1) {
-            const int k1 = R2 * m1 + t;
-            float2 z = make_float2(0.f, 0.f);
-            const int w0 = max(cp[k1], s0), w1 = min(cp[k1 + 1], s1);
-            for (int w = w0; w < w1; ++w) {
-              const float2 tw = twc[(sk0l[w - s0] * x0) % n0];
-              z = c_fmac(tw, dyn[g * chunkmax + (w - s0)], z);  // conj(e^{-i..}) * Y
-            }
-            a[m1] = z;
-          }
-          RDFT<R1, +1>::run(a);
-#pragma unroll
-          for (int k1 = 1; k1 < R1; ++k1) a[k1] = c_mulc(a[k1], twr[(t * k1) % NR]);
-#pragma unroll
-          for (int k1 = 0; k1 < R1; ++k1) xch[lidx(k1, t)] = a[k1];
-        }
-        __syncthreads();
+      // step 2: length-R2 inverse DFTs, times conj(S_c); next row's maps prefetched
+      if (t < R1) {
         float2 X[R2];
-        if (t < R1) {
 #pragma unroll
-          for (int m2 = 0; m2 < R2; ++m2) X[m2] = xch[lidx(t, m2)];
-          RDFT<R2, +1>::run(X);
-        }
-        __syncthreads();
-        // xch reused as the per-coil product buffer [G][NR]
-        if (t < R1) {
+        for (int m2 = 0; m2 < R2; ++m2) X[m2] = xch[lidx(t, m2)];
+        RDFT<R2, +1>::run(X);
 #pragma unroll
-          for (int k2 = 0; k2 < R2; ++k2) xch[g * NR + t + R1 * k2] = c_scale(c_mulc(X[k2], sv[k2]), cm);
-        }
-        __syncthreads();
-        if (tid < NR) {  // coil sum; the first contribution writes, later ones add
-          float2 s = make_float2(0.f, 0.f);
+        for (int k2 = 0; k2 < R2; ++k2) prod[g * NR + t + R1 * k2] = c_scale(c_mulc(X[k2], sv[k2]), cm);
+        if (x0 + 1 < n0) {
 #pragma unroll
-          for (int q = 0; q < G; ++q) s = c_add(s, xch[q * NR + tid]);
-          op[x0 * n1 + tid] = c_add(c_scale(s, scale), prev);
+          for (int k2 = 0; k2 < R2; ++k2) sv[k2] = sp[(x0 + 1) * n1 + t + R1 * k2];
         }
-        __syncthreads();
+      }
+      __syncthreads();
+      // coil sum (prod is rewritten only after the next row's first barrier)
+      if (tid < NR) {
+        float2 sum = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int q = 0; q < G; ++q) sum = c_add(sum, prod[q * NR + tid]);
+        sum = c_scale(sum, scale);
+        if (add_prev) sum = c_add(sum, op[x0 * n1 + tid]);
+        op[x0 * n1 + tid] = sum;
       }
     }
   }
-  if (cnt == 0 && tid < NR) {  // a shot without samples contributes zero
-    for (int r = 0; r < RB; ++r) {
-      const int x0 = row0 + r;
-      if (x0 < n0) op[x0 * n1 + tid] = make_float2(0.f, 0.f);
-    }
+}
+
+// Folds partial images parts[2..ncg) of every shot into parts 0 and 1.
+__global__ void k_fold_parts(float2* __restrict__ out, long long out_bs, int ncg, long long vol, int nbatch) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vol * nbatch) return;
+  const long long b = i / vol, e = i % vol;
+  float2* base = out + b * ncg * out_bs + e;
+  float2 p0 = base[0], p1 = base[out_bs];
+  for (int q = 2; q < ncg; ++q) {
+    const float2 v = base[q * out_bs];
+    if (q & 1) p1 = c_add(p1, v); else p0 = c_add(p0, v);
   }
+  base[0] = p0;
+  base[out_bs] = p1;
 }
 
 // Fixed-order sum of per-workgroup partials: out[b] = sum_i partial[b*nper + i]
@@ -543,6 +550,8 @@ int me_encode_forward(const void* u, long long u_bstride, const void* maps, int 
   return 0;
 }
 
+int me_encode_coil_groups(int ncoils) { return (ncoils + kCoilsPerWG - 1) / kCoilsPerWG; }
+
 int me_encode_adjoint(const void* y, const void* maps, int ncoils, int nbatch, const int* n, const int* samp_k0,
                       const int* shot_off, const int* shot_cnt, const int* colptr, void* out, long long out_bstride,
                       hipStream_t stream) {
@@ -565,11 +574,17 @@ int me_encode_adjoint(const void* y, const void* maps, int ncoils, int nbatch, c
   A.out_bs = out_bstride;
   A.ncg = (ncoils + kCoilsPerWG - 1) / kCoilsPerWG;
   const size_t shmem = (size_t)kCoilsPerWG * kAdjChunk * sizeof(float2);
-  const int nrb = (n[0] + 15) / 16;
-  dim3 grid((unsigned)(n[2] * nrb), (unsigned)nbatch);
+  dim3 grid((unsigned)(n[2] * A.ncg), (unsigned)nbatch);
   AdjLaunch f{&A, grid, shmem, stream};
   ME_REQUIRE(dispatch_line_length(n[1], f), "encode_adjoint: unsupported N1");
   ME_TRY_HIP(hipGetLastError());
+  if (A.ncg > 2) {
+    const long long vol = (long long)n[0] * n[1] * n[2];
+    const long long work = vol * nbatch;
+    hipLaunchKernelGGL(k_fold_parts, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, stream, (float2*)out,
+                       out_bstride, A.ncg, vol, nbatch);
+    ME_TRY_HIP(hipGetLastError());
+  }
   return 0;
 }
 

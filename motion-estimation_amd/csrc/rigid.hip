@@ -228,6 +228,7 @@ std::vector<Pass> build_deriv(int p) {
 struct Bufs {
   const float2* x;
   long long x_bs;
+  int x_parts;  // partial volumes per batch entry summed by the first pass (1 or 2)
   float2* y;
   long long y_bs;
   float2* w;
@@ -243,11 +244,22 @@ int run_passes(const std::vector<Pass>& passes, const VolGeom& g, Bufs b, int nb
     A.coef = coef;
     A.ncoef = kNumCoef;
     A.mode = 0;
+    int nin = p.nin;
+    HostPhase ph[2][2];
+    for (int o = 0; o < 2; ++o)
+      for (int k = 0; k < 2; ++k) ph[o][k] = p.ph[o][k];
     for (int k = 0; k < p.nin; ++k) {
       int bi = (i == 0 && p.in[k] == BY) ? BX : p.in[k];
       A.in[k] = bi == BX ? b.x : (bi == BY ? b.y : b.w);
       A.in_bs[k] = bi == BX ? b.x_bs : (bi == BY ? b.y_bs : b.w_bs);
       ME_REQUIRE(A.in[k] != nullptr, "rigid: missing buffer");
+    }
+    if (i == 0 && b.x_parts == 2) {  // sum two partial inputs inside the first pass
+      ME_REQUIRE(p.nin == 1 && p.in[0] == BY, "rigid: partial inputs need a single-input first pass");
+      nin = 2;
+      A.in[1] = b.x + (long long)g.n[0] * g.n[1] * g.n[2];  // next part volume
+      A.in_bs[1] = b.x_bs;
+      for (int o = 0; o < p.nout; ++o) ph[o][1] = ph[o][0];
     }
     for (int k = 0; k < p.nout; ++k) {
       A.out[k] = p.out[k] == BY ? b.y : b.w;
@@ -263,7 +275,7 @@ int run_passes(const std::vector<Pass>& passes, const VolGeom& g, Bufs b, int nb
       A.mask_bs = 0;
       A.accumulate = accumulate ? 1 : 0;
     }
-    int rc = launch_linepass(g, p.axis, p.nin, p.nout, A, p.ph, stream);
+    int rc = launch_linepass(g, p.axis, nin, p.nout, A, ph, stream);
     if (rc) return rc;
   }
   return 0;
@@ -301,13 +313,14 @@ const char* me_last_error(void) { return me::get_error(); }
 
 int me_version(void) { return ME_ABI_VERSION; }
 
-int me_rigid_transform(const void* x, long long x_bstride, void* y, long long y_bstride, void* work,
+int me_rigid_transform(const void* x, long long x_bstride, int x_parts, void* y, long long y_bstride, void* work,
                        long long work_bstride, int nbatch, const int* n, const long long* st, const double* r0,
                        const double* r1, const double* r2, const double* theta, long long theta_stride,
                        double* coef_ws, int mode, int flags, const unsigned char* mask, hipStream_t stream) {
   VolGeom g;
   if (int rc = make_geom(g, n, st, r0, r1, r2)) return rc;
   ME_REQUIRE(x && y, "rigid_transform: null input/output");
+  ME_REQUIRE(x_parts == 1 || x_parts == 2, "rigid_transform: x_parts must be 1 or 2");
   ME_REQUIRE(nbatch >= 1, "rigid_transform: nbatch < 1");
   ME_REQUIRE(mode >= 0 && mode <= 2, "rigid_transform: mode must be 0 (apply), 1 (adjoint), 2 (apply, inverse translation)");
   const bool sum = flags & ME_FLAG_SUM_BATCH;
@@ -316,7 +329,7 @@ int me_rigid_transform(const void* x, long long x_bstride, void* y, long long y_
   ME_REQUIRE(!(flags & ME_FLAG_ACCUMULATE) || sum, "rigid_transform: accumulate needs the batch sum");
   if (int rc = launch_coef(theta, theta_stride, coef_ws, nbatch, stream)) return rc;
   std::vector<Pass> passes = mode == 1 ? build_adjoint() : build_apply(mode == 2);
-  Bufs b{(const float2*)x, x_bstride, (float2*)(sum ? work : y), sum ? work_bstride : y_bstride, nullptr, 0};
+  Bufs b{(const float2*)x, x_bstride, x_parts, (float2*)(sum ? work : y), sum ? work_bstride : y_bstride, nullptr, 0};
   return run_passes(passes, g, b, nbatch, coef_ws, sum ? (float2*)y : nullptr,
                     (flags & ME_FLAG_MASK) ? mask : nullptr, (flags & ME_FLAG_ACCUMULATE) != 0, stream);
 }
@@ -331,7 +344,7 @@ int me_rigid_deriv(const void* x, long long x_bstride, void* y, long long y_bstr
   ME_REQUIRE(p_idx >= 0 && p_idx <= 5, "rigid_deriv: p_idx must be in [0, 5]");
   ME_REQUIRE(p_idx < 3 || work, "rigid_deriv: rotation derivatives need a work buffer");
   if (int rc = launch_coef(theta, theta_stride, coef_ws, nbatch, stream)) return rc;
-  Bufs b{(const float2*)x, x_bstride, (float2*)y, y_bstride, (float2*)work, work_bstride};
+  Bufs b{(const float2*)x, x_bstride, 1, (float2*)y, y_bstride, (float2*)work, work_bstride};
   return run_passes(build_deriv(p_idx), g, b, nbatch, coef_ws, nullptr, nullptr, false, stream);
 }
 

@@ -188,7 +188,8 @@ class SenseProblem:
         self.M = real_to_plane(M, self.shape, torch.uint8, dev) if M is not None else Mp
         # batching over shots
         budget = mem_budget or int(8 * 2**30)
-        self.batch = int(max(1, min(self.nshots, budget // max(1, vol * 8))))
+        ncg = (self.ncoils + 15) // 16
+        self.batch = int(max(1, min(self.nshots, budget // max(1, vol * 8 * ncg))))
         self.theta = torch.zeros((self.nshots, 6), dtype=torch.float64, device=dev)
         self.damp = torch.ones(self.nshots, dtype=torch.float64, device=dev)
         self.flags = torch.zeros(self.nshots, dtype=torch.uint8, device=dev)
@@ -196,6 +197,8 @@ class SenseProblem:
         self._W = None
         self._Y = None
         self._rhs_cache = None
+        self.ncg = int(_lib.size("me_encode_coil_groups", self.ncoils))  # adjoint partial images per shot
+        self.parts = min(self.ncg, 2)  # after folding, summed by the first adjoint pass
         self._vec_partials = torch.empty(_lib.size("me_vec_partials_size"), dtype=torch.float64, device=dev)
         self._cg_state = torch.zeros(8, dtype=torch.float64, device=dev)
         nps = _lib.size("me_encode_partials_size", n2, self.ncoils, self.plan.max_cnt, n1)
@@ -208,11 +211,13 @@ class SenseProblem:
                   _ll4((0, 0, 0, 1)), _i4((1, 1, 1, n)), _i4((0, 0, 0, 0)), float(scale), _lib.stream(t.device))
 
     def _ubuf(self):
+        """Batch buffer: B transformed images, or B x ngroups adjoint partials."""
         if self._U is None:
-            self._U = torch.empty((self.batch, self.vol), dtype=torch.complex64, device=self.device)
+            self._U = torch.empty((self.batch * self.ncg, self.vol), dtype=torch.complex64, device=self.device)
         return self._U
 
     def _wbuf(self):
+        """Second batch buffer (B images): work volumes of the transforms."""
         if self._W is None:
             self._W = torch.empty((self.batch, self.vol), dtype=torch.complex64, device=self.device)
         return self._W
@@ -230,11 +235,12 @@ class SenseProblem:
         return torch.zeros(self.vol, dtype=torch.complex64, device=self.device)
 
     # ------------------------------------------------------------ kernels
-    def _rigid(self, x, x_bs, out, theta, mode, flags=0, work=None, mask=None):
+    def _rigid(self, x, x_bs, out, theta, mode, flags=0, work=None, mask=None, x_parts=1):
         g = self.geom
         B = theta.shape[0]
         coef = torch.empty((B, 16), dtype=torch.float64, device=self.device)
-        _lib.call("me_rigid_transform", _lib.ptr(x), x_bs, _lib.ptr(out), self.vol, _lib.ptr(work), self.vol, int(B),
+        _lib.call("me_rigid_transform", _lib.ptr(x), x_bs, int(x_parts), _lib.ptr(out), self.vol, _lib.ptr(work),
+                  self.vol, int(B),
                   g.n_arg(), g.st_arg(True), *g.r_args(), _lib.ptr(theta), int(theta.stride(0)), _lib.ptr(coef),
                   int(mode), int(flags), _lib.ptr(mask), _lib.stream(self.device))
 
@@ -272,6 +278,12 @@ class SenseProblem:
             self._encode(U, self.vol, s0, s1, y=y, meas=meas)
         return y
 
+    def _adjoint_sum(self, parts, s0, s1, out, theta):
+        """out (+)= M * sum_{s in batch} T_s^H (sum of the shot's partial images)."""
+        flags = 1 | 2 | (4 if s0 > 0 else 0)
+        self._rigid(parts, self.vol * self.ncg, out, theta[s0:s1], 1, flags=flags, work=self._wbuf(), mask=self.M,
+                    x_parts=self.parts)
+
     def normal(self, p, out, theta=None):
         """out = M * sum_s T_s^H E_s^H E_s T_s p  (image_estimation.py:106-129)."""
         theta = self.theta if theta is None else theta
@@ -281,7 +293,7 @@ class SenseProblem:
             self._rigid(p, 0, U, theta[s0:s1], 0)
             self._encode(U, self.vol, s0, s1, y=Y)
             self._decode(Y, s0, s1, U)
-            self._rigid(U, self.vol, out, theta[s0:s1], 1, flags=1 | 2 | (4 if s0 > 0 else 0), work=U, mask=self.M)
+            self._adjoint_sum(U, s0, s1, out, theta)
         return out
 
     def rhs(self, theta=None, out=None, cache=True):
@@ -289,20 +301,19 @@ class SenseProblem:
         E_s^H y_s does not depend on the motion and is cached when it fits."""
         theta = self.theta if theta is None else theta
         out = self.zeros_image() if out is None else out
-        if cache and self._rhs_cache is None and self.nshots * self.vol * 8 <= int(16 * 2**30):
-            self._rhs_cache = torch.empty((self.nshots, self.vol), dtype=torch.complex64, device=self.device)
+        if cache and self._rhs_cache is None and self.nshots * self.ncg * self.vol * 8 <= int(16 * 2**30):
+            self._rhs_cache = torch.empty((self.nshots * self.ncg, self.vol), dtype=torch.complex64,
+                                          device=self.device)
             for s0, s1 in self.batches():
-                self._decode(self.ymeas, s0, s1, self._rhs_cache[s0:s1])
+                self._decode(self.ymeas, s0, s1, self._rhs_cache[s0 * self.ncg:s1 * self.ncg])
         U = self._ubuf()
         for s0, s1 in self.batches():
-            if self._rhs_cache is not None:
-                src = self._rhs_cache[s0:s1]
-                self._rigid(src, self.vol, out, theta[s0:s1], 1, flags=1 | 2 | (4 if s0 > 0 else 0), work=U,
-                            mask=self.M)
+            if cache and self._rhs_cache is not None:
+                src = self._rhs_cache[s0 * self.ncg:s1 * self.ncg]
             else:
                 self._decode(self.ymeas, s0, s1, U)
-                self._rigid(U, self.vol, out, theta[s0:s1], 1, flags=1 | 2 | (4 if s0 > 0 else 0), work=U,
-                            mask=self.M)
+                src = U
+            self._adjoint_sum(src, s0, s1, out, theta)
         return out
 
     def cg(self, x, b, max_iter, tol=1e-6, check_every=1, theta=None):

@@ -101,7 +101,7 @@ def generate_corrupted_kspace(gt_img, mps, A, transforms, batch=None):
     for s0 in range(0, S, B):
         s1 = min(S, s0 + B)
         coef = torch.empty((s1 - s0, 16), dtype=torch.float64, device=dev)
-        _lib.call("me_rigid_transform", _lib.ptr(xp), 0, _lib.ptr(U), vol, None, 0, s1 - s0, geom.n_arg(),
+        _lib.call("me_rigid_transform", _lib.ptr(xp), 0, 1, _lib.ptr(U), vol, None, 0, s1 - s0, geom.n_arg(),
                   geom.st_arg(True), *geom.r_args(), _lib.ptr(theta[s0:s1]), 6, _lib.ptr(coef), 0, 0, None, s)
         maxc = int(plan.cnt_h[s0:s1].max())
         _lib.call("me_encode_forward", _lib.ptr(U), vol, _lib.ptr(mp), ncoils, s1 - s0, _lib.int3(shape),

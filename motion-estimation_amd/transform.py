@@ -70,7 +70,7 @@ def rigid_batch(x, theta, geom, mode=0, out=None, plane=False, sum_batch=False, 
         work = torch.empty((B, vol), dtype=torch.complex64, device=x.device)
     coef = torch.empty((B, 16), dtype=torch.float64, device=x.device)
     flags = (1 if sum_batch else 0) | (2 if mask is not None else 0)
-    _lib.call("me_rigid_transform", _lib.ptr(x), vol if x.numel() > vol else 0, _lib.ptr(out), vol,
+    _lib.call("me_rigid_transform", _lib.ptr(x), vol if x.numel() > vol else 0, 1, _lib.ptr(out), vol,
               _lib.ptr(work), vol, int(B), geom.n_arg(), geom.st_arg(plane), *geom.r_args(), _lib.ptr(theta),
               int(theta.stride(0)), _lib.ptr(coef), int(mode), int(flags), _lib.ptr(mask), _lib.stream(x.device))
     return out

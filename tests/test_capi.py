@@ -41,7 +41,7 @@ def test_argument_validation_without_device(me):
     n = me._lib.int3((16, 16, 16))
     st = me._lib.ll3((256, 16, 1))
     # null input -> error code and message, no device work
-    rc = lib.me_rigid_transform(None, 0, None, 0, None, 0, 1, n, st, None, None, None, None, 6, None, 0, 0,
+    rc = lib.me_rigid_transform(None, 0, 1, None, 0, None, 0, 1, n, st, None, None, None, None, 6, None, 0, 0,
                                 None, None)
     assert rc != 0
     assert b"null" in lib.me_last_error()

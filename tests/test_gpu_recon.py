@@ -96,8 +96,9 @@ def test_encoding_adjoint_identity(me, prob_data):
     Y = torch.empty_like(pb.ymeas)
     pb._encode(U, pb.vol, 0, pb.nshots, y=Y)
     y2 = torch.randn_like(Y)
-    W = torch.empty_like(U)
+    W = torch.empty((pb.nshots * pb.ncg, pb.vol), dtype=torch.complex64, device=U.device)
     pb._decode(y2, 0, pb.nshots, W)
+    W = W.reshape(pb.nshots, pb.ncg, pb.vol)[:, :pb.parts].sum(1)  # parts 0 and 1 hold the sum
     lhs = torch.vdot(Y, y2).cpu().numpy()
     rhs = sum(torch.vdot(U[s], W[s]).cpu().numpy() for s in range(pb.nshots))
     assert abs(lhs - rhs) / abs(lhs) < 1e-5
@@ -177,3 +178,31 @@ def test_joint_recon_matches_oracle(me, prob_data):
     assert _motion_err(t_got, t_ref) < MOTION_TOL
     assert np.allclose(app.objective_history, ref.objective_history, rtol=TOL, atol=0)
     assert app.transform_history.shape == ref.transform_history.shape
+
+
+@pytest.mark.parametrize("ncoils", [20, 40])
+def test_normal_op_many_coil_groups(me, ncoils):
+    """Coil counts above 16 split the adjoint into coil-group partial images
+    (two summed in the first adjoint pass, more folded first)."""
+    shape = (16, 16, 8)
+    sim = me.simulation
+    mps = sim.coil_maps(shape, ncoils, seed=4).numpy().astype(np.complex128)
+    A, _ = sim.disorder_sampling(shape, (4, 4), 2)
+    t = sim.motion_uniform(A.shape[0], rot_deg=3.0, trans_vox=1.0, seed=5)
+    rng = np.random.default_rng(8)
+    ksp = rng.standard_normal((ncoils, *shape)) + 1j * rng.standard_normal((ncoils, *shape))
+    A_u = np.fft.ifftshift(A, axes=(1, 2))
+    kg, rkg = me.compute_transform_grids_voxel(shape, (1, 1, 1))
+    okg, orkg = O.compute_transform_grids_voxel(shape, (1, 1, 1))
+    pb = me.encoding.SenseProblem(ksp, mps, A_u, kg, rkg)
+    assert pb.ncg == (ncoils + 15) // 16
+    pb.theta.copy_(torch.as_tensor(t))
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    out = pb.zeros_image()
+    pb.normal(pb.image_to_plane(x), out)
+    M = np.sum(np.abs(mps) ** 2, axis=0) > 0.1
+    ref = O.normal_op(x, mps, A_u, t, okg, orkg, M)
+    assert _rel(pb.image_from_plane(out), ref) < TOL
+    refb = O.cg_rhs(ksp, mps, A_u, t, okg, orkg, M)
+    assert _rel(pb.image_from_plane(pb.rhs(cache=True)), refb) < TOL
+    assert _rel(pb.image_from_plane(pb.rhs(cache=False)), refb) < TOL
