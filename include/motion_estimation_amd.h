@@ -99,15 +99,13 @@ int me_encode_forward(const void* u, long long u_bstride, const void* maps, int 
                       const int* samp_k0, const int* samp_k1, const int* shot_off, const int* shot_cnt, int max_cnt,
                       void* y, const void* y_meas, double* partials, double* norms, hipStream_t stream);
 
-/* Number of coil groups (16 coils each) the adjoint splits the coils into. */
+/* Number of partial images per shot written by me_encode_adjoint (1: the
+ * adjoint sums all coils itself). */
 int me_encode_coil_groups(int ncoils);
 
-/* Adjoint of me_encode_forward for a batch of shots, per coil group g:
- *   part_{s,g} = sum_{c in g} conj(S_c) F_01^H (M_s Y_{s,c})   (image_estimation.py:94-102,
- *   before T^H); sum_g part_{s,g} is E_s^H Y_s.  Part g of shot s is written at
- *   out + (s*ngroups + g)*out_bstride (plane-major); with more than two groups,
- *   groups 2.. are folded into parts 0 and 1, so the sum of parts 0 and 1 is the
- *   result (me_rigid_transform's x_parts = min(ngroups, 2) consumes them). */
+/* Adjoint of me_encode_forward for a batch of shots:
+ *   out_s = sum_c conj(S_c) F_01^H (M_s Y_{s,c})   (image_estimation.py:94-102,
+ *   before T^H), written at out + s*out_bstride (plane-major). */
 int me_encode_adjoint(const void* y, const void* maps, int ncoils, int nbatch, const int* n, const int* samp_k0,
                       const int* shot_off, const int* shot_cnt, const int* colptr, void* out, long long out_bstride,
                       hipStream_t stream);

@@ -205,35 +205,54 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
   }
 }
 
-// Adjoint.  One workgroup per (readout plane x2, coil group, shot): the
-// group's samples are staged in LDS once, then all N0 rows are streamed:
+// (k*x) mod n for 0 <= k, x < n <= 512: a mask when n is a power of two,
+// else one float-reciprocal quotient estimate and a single correction.
+__device__ __forceinline__ int mulmod(int k, int x, int n, int mask, float inv_n) {
+  const int p = k * x;
+  if (mask >= 0) return p & mask;
+  int q = (int)((float)p * inv_n);
+  int r = p - q * n;
+  r += (r < 0) ? n : 0;
+  r -= (r >= n) ? n : 0;
+  return r;
+}
+
+// Adjoint.  One workgroup per (readout plane x2, block of 16 rows, shot);
+// each row is one line of T threads (the line-pass layout) and the workgroup
+// loops over the coils, accumulating the coil sum of its rows in registers:
 //   Z_c(x0, k1) = sum_{k0 in column k1} e^{+2 pi i k0 x0/N0} Y_c(k0, k1)   (scatter)
 //   z_c(x0, :)  = IFFT_1 Z_c(x0, :)                                         (row IFFT)
-//   part(x0, :) = sum_{c in group} conj(S_c(x0, :)) z_c(x0, :)              (coil sum)
-// Each coil group writes its own partial image (part cg of the shot); the
-// adjoint rigid transform's first pass sums the parts.
+//   out(x0, :) += conj(S_c(x0, :)) z_c(x0, :)                              (coil sum)
+// A coil's samples (one coil-plane of the gathered data) are staged in LDS,
+// the next coil's samples and map rows are prefetched into registers while
+// the current coil is transformed.
 template <int NR>
 __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
   constexpr int R1 = RowT<NR>::R1, R2 = RowT<NR>::R2, T = RowT<NR>::T, NT = RowT<NR>::NT;
-  constexpr int G = kCoilsPerWG;
-  extern __shared__ float2 dyn[];  // [G][kAdjChunk] samples of the coil group
-  __shared__ float2 xch[G * R1 * (R2 + 1)];
-  __shared__ float2 prod[G * NR];
+  constexpr int L = kCoilsPerWG;  // rows per workgroup
+  constexpr int SPT = (kAdjChunk + NT - 1) / NT;  // staged samples per thread
+  __shared__ float2 yc[2][kAdjChunk];
+  __shared__ int sk0l[kAdjChunk];
+  __shared__ float2 xch[L * R1 * (R2 + 1)];
   __shared__ float2 twr[NR];
   __shared__ float2 twc[kMaxCol];
   __shared__ int cp[NR + 1];
-  __shared__ int sk0l[kAdjChunk];
 
   const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
   const int b = blockIdx.y;
+  const int nrb = (n0 + L - 1) / L;
   const int x2 = blockIdx.x % n2;
-  const int cg = blockIdx.x / n2;
+  const int row0 = (blockIdx.x / n2) * L;
   const int tid = threadIdx.x;
-  const int t = tid % T, g = tid / T;
+  const int t = tid % T, r = tid / T;
+  const int x0 = row0 + r;
+  const bool rvalid = x0 < n0;
+  const int xr = rvalid ? x0 : 0;
   const int cnt = A.cnt[b], off = A.off[b];
-  const int c = cg * G + g;
-  const bool cvalid = c < A.ncoils;
-  const float cm = cvalid ? 1.f : 0.f;
+  const int C = A.ncoils;
+  const int mask = (n0 & (n0 - 1)) == 0 ? n0 - 1 : -1;
+  const float inv_n0 = 1.0f / (float)n0;
+  (void)nrb;
 
   fill_twiddles<NR, NT>(twr);
   for (int j = tid; j < n0; j += NT) {
@@ -246,37 +265,47 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
 
   const long long plane = (long long)n0 * n1;
   const long long vol = plane * n2;
-  const long long ybase = (long long)off * n2 * A.ncoils;
+  const long long ybase = (long long)off * n2 * C + (long long)x2 * C * cnt;  // coil c at ybase + c*cnt
   const float scale = rsqrtf((float)n0 * (float)n1);
-  const float2* sp = A.maps + (long long)(cvalid ? c : 0) * vol + (long long)x2 * plane;
-  float2* op = A.out + ((long long)b * A.ncg + cg) * A.out_bs + (long long)x2 * plane;
-  auto lidx = [&](int k1, int m2) { return g * (R1 * (R2 + 1)) + k1 * (R2 + 1) + m2; };
+  const float2* srow = A.maps + (long long)x2 * plane + (long long)xr * n1;  // coil c at srow + c*vol
+  auto lidx = [&](int k1, int m2) { return r * (R1 * (R2 + 1)) + k1 * (R2 + 1) + m2; };
 
-  if (cnt == 0) {  // a shot without samples contributes zero
-    for (long long j = tid; j < plane; j += NT) op[j] = make_float2(0.f, 0.f);
-    return;
-  }
+  float2 acc[R2];
+#pragma unroll
+  for (int k2 = 0; k2 < R2; ++k2) acc[k2] = make_float2(0.f, 0.f);
+
   for (int s0 = 0; s0 < cnt; s0 += kAdjChunk) {
     const int ns = min(kAdjChunk, cnt - s0);
-    __syncthreads();  // previous chunk's readers are done
-#pragma unroll 8
-    for (int j = tid; j < G * kAdjChunk; j += NT) {
-      const int q = j / kAdjChunk, w = j % kAdjChunk;
-      const int cc = cg * G + q;
-      const bool ok = cc < A.ncoils && w < ns;
-      const long long src = ybase + ((long long)x2 * A.ncoils + (ok ? cc : 0)) * cnt + s0 + (ok ? w : 0);
-      const float2 v = A.y[src];
-      dyn[j] = ok ? v : make_float2(0.f, 0.f);
-    }
+    __syncthreads();  // readers of the previous chunk's tables are done
     for (int j = tid; j < ns; j += NT) sk0l[j] = A.sk0[off + s0 + j];
-    __syncthreads();
-    const bool add_prev = s0 > 0;
+    // the thread's column ranges within this chunk (column k1 = R2*m1 + t)
+    // prefetch coil 0's samples and map row
+    float2 ypre[SPT];
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+      const int w = tid + j * NT;
+      ypre[j] = A.y[ybase + s0 + (w < ns ? w : 0)];
+    }
     float2 sv[R2];
     if (t < R1) {
 #pragma unroll
-      for (int k2 = 0; k2 < R2; ++k2) sv[k2] = sp[t + R1 * k2];
+      for (int k2 = 0; k2 < R2; ++k2) sv[k2] = srow[t + R1 * k2];
     }
-    for (int x0 = 0; x0 < n0; ++x0) {
+    for (int c = 0; c < C; ++c) {
+      float2* ycur = yc[c & 1];
+#pragma unroll
+      for (int j = 0; j < SPT; ++j) {
+        const int w = tid + j * NT;
+        if (w < ns) ycur[w] = ypre[j];
+      }
+      if (c + 1 < C) {  // next coil's samples, in flight during this coil
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) {
+          const int w = tid + j * NT;
+          ypre[j] = A.y[ybase + (long long)(c + 1) * cnt + s0 + (w < ns ? w : 0)];
+        }
+      }
+      __syncthreads();
       // step 1: scatter this row's spectrum, length-R1 inverse DFTs
       if (t < R2) {
         float2 a[R1];
@@ -285,7 +314,7 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
           const int k1 = R2 * m1 + t;
           float2 z = make_float2(0.f, 0.f);
           const int w0 = max(cp[k1], s0) - s0, w1 = min(cp[k1 + 1], s0 + ns) - s0;
-          for (int w = w0; w < w1; ++w) z = c_fmac(twc[(sk0l[w] * x0) % n0], dyn[g * kAdjChunk + w], z);
+          for (int w = w0; w < w1; ++w) z = c_fmac(twc[mulmod(sk0l[w], xr, n0, mask, inv_n0)], ycur[w], z);
           a[m1] = z;
         }
         RDFT<R1, +1>::run(a);
@@ -295,46 +324,27 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
         for (int k1 = 0; k1 < R1; ++k1) xch[lidx(k1, t)] = a[k1];
       }
       __syncthreads();
-      // step 2: length-R2 inverse DFTs, times conj(S_c); next row's maps prefetched
+      // step 2: length-R2 inverse DFTs, times conj(S_c), accumulated
       if (t < R1) {
         float2 X[R2];
 #pragma unroll
         for (int m2 = 0; m2 < R2; ++m2) X[m2] = xch[lidx(t, m2)];
         RDFT<R2, +1>::run(X);
 #pragma unroll
-        for (int k2 = 0; k2 < R2; ++k2) prod[g * NR + t + R1 * k2] = c_scale(c_mulc(X[k2], sv[k2]), cm);
-        if (x0 + 1 < n0) {
+        for (int k2 = 0; k2 < R2; ++k2) acc[k2] = c_add(acc[k2], c_mulc(X[k2], sv[k2]));
+        if (c + 1 < C) {
+          const float2* sn = srow + (long long)(c + 1) * vol;
 #pragma unroll
-          for (int k2 = 0; k2 < R2; ++k2) sv[k2] = sp[(x0 + 1) * n1 + t + R1 * k2];
+          for (int k2 = 0; k2 < R2; ++k2) sv[k2] = sn[t + R1 * k2];
         }
-      }
-      __syncthreads();
-      // coil sum (prod is rewritten only after the next row's first barrier)
-      if (tid < NR) {
-        float2 sum = make_float2(0.f, 0.f);
-#pragma unroll
-        for (int q = 0; q < G; ++q) sum = c_add(sum, prod[q * NR + tid]);
-        sum = c_scale(sum, scale);
-        if (add_prev) sum = c_add(sum, op[x0 * n1 + tid]);
-        op[x0 * n1 + tid] = sum;
       }
     }
   }
-}
-
-// Folds partial images parts[2..ncg) of every shot into parts 0 and 1.
-__global__ void k_fold_parts(float2* __restrict__ out, long long out_bs, int ncg, long long vol, int nbatch) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= vol * nbatch) return;
-  const long long b = i / vol, e = i % vol;
-  float2* base = out + b * ncg * out_bs + e;
-  float2 p0 = base[0], p1 = base[out_bs];
-  for (int q = 2; q < ncg; ++q) {
-    const float2 v = base[q * out_bs];
-    if (q & 1) p1 = c_add(p1, v); else p0 = c_add(p0, v);
+  if (t < R1 && rvalid) {
+    float2* op = A.out + (long long)b * A.out_bs + (long long)x2 * plane + (long long)x0 * n1;
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) op[t + R1 * k2] = c_scale(acc[k2], scale);
   }
-  base[0] = p0;
-  base[out_bs] = p1;
 }
 
 // Fixed-order sum of per-workgroup partials: out[b] = sum_i partial[b*nper + i]
@@ -550,7 +560,7 @@ int me_encode_forward(const void* u, long long u_bstride, const void* maps, int 
   return 0;
 }
 
-int me_encode_coil_groups(int ncoils) { return (ncoils + kCoilsPerWG - 1) / kCoilsPerWG; }
+int me_encode_coil_groups(int ncoils) { return ncoils >= 1 ? 1 : 0; }
 
 int me_encode_adjoint(const void* y, const void* maps, int ncoils, int nbatch, const int* n, const int* samp_k0,
                       const int* shot_off, const int* shot_cnt, const int* colptr, void* out, long long out_bstride,
@@ -572,19 +582,12 @@ int me_encode_adjoint(const void* y, const void* maps, int ncoils, int nbatch, c
   A.colptr = colptr;
   A.out = (float2*)out;
   A.out_bs = out_bstride;
-  A.ncg = (ncoils + kCoilsPerWG - 1) / kCoilsPerWG;
-  const size_t shmem = (size_t)kCoilsPerWG * kAdjChunk * sizeof(float2);
-  dim3 grid((unsigned)(n[2] * A.ncg), (unsigned)nbatch);
-  AdjLaunch f{&A, grid, shmem, stream};
+  A.ncg = 1;
+  const int nrb = (n[0] + kCoilsPerWG - 1) / kCoilsPerWG;
+  dim3 grid((unsigned)(n[2] * nrb), (unsigned)nbatch);
+  AdjLaunch f{&A, grid, 0, stream};
   ME_REQUIRE(dispatch_line_length(n[1], f), "encode_adjoint: unsupported N1");
   ME_TRY_HIP(hipGetLastError());
-  if (A.ncg > 2) {
-    const long long vol = (long long)n[0] * n[1] * n[2];
-    const long long work = vol * nbatch;
-    hipLaunchKernelGGL(k_fold_parts, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, stream, (float2*)out,
-                       out_bstride, A.ncg, vol, nbatch);
-    ME_TRY_HIP(hipGetLastError());
-  }
   return 0;
 }
 

@@ -188,7 +188,7 @@ class SenseProblem:
         self.M = real_to_plane(M, self.shape, torch.uint8, dev) if M is not None else Mp
         # batching over shots
         budget = mem_budget or int(8 * 2**30)
-        ncg = (self.ncoils + 15) // 16
+        ncg = int(_lib.size("me_encode_coil_groups", self.ncoils))
         self.batch = int(max(1, min(self.nshots, budget // max(1, vol * 8 * ncg))))
         self.theta = torch.zeros((self.nshots, 6), dtype=torch.float64, device=dev)
         self.damp = torch.ones(self.nshots, dtype=torch.float64, device=dev)

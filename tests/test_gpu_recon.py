@@ -182,8 +182,8 @@ def test_joint_recon_matches_oracle(me, prob_data):
 
 @pytest.mark.parametrize("ncoils", [20, 40])
 def test_normal_op_many_coil_groups(me, ncoils):
-    """Coil counts above 16 split the adjoint into coil-group partial images
-    (two summed in the first adjoint pass, more folded first)."""
+    """Coil counts that are not multiples of the 16-coil forward groups, and
+    more than two groups."""
     shape = (16, 16, 8)
     sim = me.simulation
     mps = sim.coil_maps(shape, ncoils, seed=4).numpy().astype(np.complex128)
@@ -195,7 +195,6 @@ def test_normal_op_many_coil_groups(me, ncoils):
     kg, rkg = me.compute_transform_grids_voxel(shape, (1, 1, 1))
     okg, orkg = O.compute_transform_grids_voxel(shape, (1, 1, 1))
     pb = me.encoding.SenseProblem(ksp, mps, A_u, kg, rkg)
-    assert pb.ncg == (ncoils + 15) // 16
     pb.theta.copy_(torch.as_tensor(t))
     x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
     out = pb.zeros_image()
