@@ -61,6 +61,7 @@ struct EncArgs {
   long long out_bs;
   int rows_per_wg;
   int max_cnt;
+  int nbatch;
 };
 
 __device__ __forceinline__ double block_sum_f64(double v, double* sh) {
@@ -89,8 +90,11 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
   __shared__ double red[16];
 
   const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
-  const int b = blockIdx.y;
+  // block index: shot fastest, so the workgroups of all shots for one
+  // (plane, coil group) run together and share the map rows through L2
   int bx = blockIdx.x;
+  const int b = bx % A.nbatch;
+  bx /= A.nbatch;
   const int x2 = bx % n2;
   bx /= n2;
   const int cg = bx % A.ncg;
@@ -201,7 +205,7 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
   }
   if (A.partial) {
     const double s = block_sum_f64(local, red);
-    if (tid == 0) A.partial[(long long)b * A.blocks_per_shot + blockIdx.x] = s;
+    if (tid == 0) A.partial[(long long)b * A.blocks_per_shot + blockIdx.x / A.nbatch] = s;
   }
 }
 
@@ -226,7 +230,7 @@ __device__ __forceinline__ int mulmod(int k, int x, int n, int mask, float inv_n
 // A coil's samples (one coil-plane of the gathered data) are staged in LDS,
 // the next coil's samples and map rows are prefetched into registers while
 // the current coil is transformed.
-template <int NR>
+template <int NR, bool POW2>
 __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
   constexpr int R1 = RowT<NR>::R1, R2 = RowT<NR>::R2, T = RowT<NR>::T, NT = RowT<NR>::NT;
   constexpr int L = kCoilsPerWG;  // rows per workgroup
@@ -239,10 +243,10 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
   __shared__ int cp[NR + 1];
 
   const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
-  const int b = blockIdx.y;
   const int nrb = (n0 + L - 1) / L;
-  const int x2 = blockIdx.x % n2;
-  const int row0 = (blockIdx.x / n2) * L;
+  const int b = blockIdx.x % A.nbatch;  // shot fastest (map rows shared through L2)
+  const int x2 = (blockIdx.x / A.nbatch) % n2;
+  const int row0 = (blockIdx.x / A.nbatch / n2) * L;
   const int tid = threadIdx.x;
   const int t = tid % T, r = tid / T;
   const int x0 = row0 + r;
@@ -250,7 +254,7 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
   const int xr = rvalid ? x0 : 0;
   const int cnt = A.cnt[b], off = A.off[b];
   const int C = A.ncoils;
-  const int mask = (n0 & (n0 - 1)) == 0 ? n0 - 1 : -1;
+  const int mask = POW2 ? n0 - 1 : -1;
   const float inv_n0 = 1.0f / (float)n0;
   (void)nrb;
 
@@ -291,6 +295,18 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
 #pragma unroll
       for (int k2 = 0; k2 < R2; ++k2) sv[k2] = srow[t + R1 * k2];
     }
+    __syncthreads();  // sk0l and cp visible
+    // the thread's sample ranges within this chunk for its columns k1 = R2*m1 + t
+    // (the same for every coil)
+    int wlo[R1], whi[R1];
+    if (t < R2) {
+#pragma unroll
+      for (int m1 = 0; m1 < R1; ++m1) {
+        const int k1 = R2 * m1 + t;
+        wlo[m1] = max(cp[k1], s0) - s0;
+        whi[m1] = min(cp[k1 + 1], s0 + ns) - s0;
+      }
+    }
     for (int c = 0; c < C; ++c) {
       float2* ycur = yc[c & 1];
 #pragma unroll
@@ -311,10 +327,12 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
         float2 a[R1];
 #pragma unroll
         for (int m1 = 0; m1 < R1; ++m1) {
-          const int k1 = R2 * m1 + t;
           float2 z = make_float2(0.f, 0.f);
-          const int w0 = max(cp[k1], s0) - s0, w1 = min(cp[k1 + 1], s0 + ns) - s0;
-          for (int w = w0; w < w1; ++w) z = c_fmac(twc[mulmod(sk0l[w], xr, n0, mask, inv_n0)], ycur[w], z);
+          for (int w = wlo[m1]; w < whi[m1]; ++w) {
+            const int kx = sk0l[w] * xr;
+            const int ti = POW2 ? (kx & mask) : mulmod(sk0l[w], xr, n0, mask, inv_n0);
+            z = c_fmac(twc[ti], ycur[w], z);
+          }
           a[m1] = z;
         }
         RDFT<R1, +1>::run(a);
@@ -490,7 +508,10 @@ struct AdjLaunch {
   hipStream_t s;
   template <int NR>
   void operator()() {
-    hipLaunchKernelGGL((k_encode_adj<NR>), grid, dim3(RowT<NR>::NT), shmem, s, *a);
+    if ((a->n0 & (a->n0 - 1)) == 0)
+      hipLaunchKernelGGL((k_encode_adj<NR, true>), grid, dim3(RowT<NR>::NT), shmem, s, *a);
+    else
+      hipLaunchKernelGGL((k_encode_adj<NR, false>), grid, dim3(RowT<NR>::NT), shmem, s, *a);
   }
 };
 struct RowThreads {
@@ -549,7 +570,9 @@ int me_encode_forward(const void* u, long long u_bstride, const void* maps, int 
   const int spt = 2;
   A.nchunk = max_cnt > 0 ? (max_cnt + spt * nt - 1) / (spt * nt) : 1;
   A.blocks_per_shot = n[2] * A.ncg * A.nchunk;
-  dim3 grid((unsigned)A.blocks_per_shot, (unsigned)nbatch);
+  A.nbatch = nbatch;
+  ME_REQUIRE((long long)A.blocks_per_shot * nbatch < (1LL << 31), "encode_forward: grid too large");
+  dim3 grid((unsigned)(A.blocks_per_shot * nbatch), 1u);
   FwdLaunch<2> f{&A, grid, stream};
   ME_REQUIRE(dispatch_line_length(n[1], f), "encode_forward: unsupported N1");
   ME_TRY_HIP(hipGetLastError());
@@ -583,8 +606,10 @@ int me_encode_adjoint(const void* y, const void* maps, int ncoils, int nbatch, c
   A.out = (float2*)out;
   A.out_bs = out_bstride;
   A.ncg = 1;
+  A.nbatch = nbatch;
   const int nrb = (n[0] + kCoilsPerWG - 1) / kCoilsPerWG;
-  dim3 grid((unsigned)(n[2] * nrb), (unsigned)nbatch);
+  ME_REQUIRE((long long)n[2] * nrb * nbatch < (1LL << 31), "encode_adjoint: grid too large");
+  dim3 grid((unsigned)(n[2] * nrb * nbatch), 1u);
   AdjLaunch f{&A, grid, 0, stream};
   ME_REQUIRE(dispatch_line_length(n[1], f), "encode_adjoint: unsupported N1");
   ME_TRY_HIP(hipGetLastError());
