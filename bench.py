@@ -6,7 +6,9 @@ shots (tile 16 x 16, acceleration 2), k-space corrupted by 128 motion states
 of smooth cubic trajectories (+-5 deg, +-5 voxels) plus 30 dB noise, all
 synthetic and seeded.  One step = one outer iteration of JointRecon: right-hand
 side, initial residual + 3 CG iterations, one LM update of all 64 x 6
-parameters with mean re-registration, x *= M, and the objective.
+parameters with mean re-registration, x *= M, and one objective value (that
+of the previous iteration's final state, taken from the CG's first encoding
+of that same state).
 
 Prints ONE JSON line with the throughput, a roofline entry for the dominant
 kernel (timed live with HIP events on the stream it runs on) and a CPU
@@ -182,11 +184,15 @@ def main():
     torch.cuda.synchronize()
     log(f"setup done in {time.perf_counter() - t_setup:.1f}s; shot batch {app.problem.batch}")
 
+    objectives = []
+
     def step():
-        app.step()
-        if not args.no_objective:
-            app.objective_history = getattr(app, "objective_history", [])
-            app.objective_history.append(app.objective())
+        # every step also yields one objective value: that of the state it
+        # starts from, out of the CG's first encoding of that state
+        buf = None if args.no_objective else torch.empty(nshots, dtype=torch.float64, device=dev)
+        app.step(prev_objective=buf)
+        if buf is not None:
+            objectives.append(buf)
 
     for i in range(args.warmup):
         step()

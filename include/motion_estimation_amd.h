@@ -68,6 +68,19 @@ int me_rigid_deriv(const void* x, long long x_bstride, void* y, long long y_bstr
                    const double* r1, const double* r2, const double* theta, long long theta_stride,
                    double* coef_ws, int p_idx, hipStream_t stream);
 
+/* The six derivative columns of me_rigid_deriv with the rotation chain shared:
+ * c1 = Rz x, c2 = Ry c1, c3 = Rx c2 (one volume per batch entry each, at
+ * c_bstride) are computed when build != 0 (call p_idx = 0 first with build = 1,
+ * then p_idx = 1..5 with build = 0 and the same theta); column p_idx is then
+ * produced from the shortest chain level that contains it.  Same result as
+ * me_rigid_deriv (transform.py:476-504) with 42 instead of 66 line passes per
+ * six columns. */
+int me_rigid_deriv_chain(const void* x, long long x_bstride, void* c1, void* c2, void* c3, long long c_bstride,
+                         void* y, long long y_bstride, void* work, long long work_bstride, int nbatch, const int* n,
+                         const long long* st, const double* r0, const double* r1, const double* r2,
+                         const double* theta, long long theta_stride, double* coef_ws, int p_idx, int build,
+                         hipStream_t stream);
+
 /* Orthonormal 1-D FFT (inverse != 0: inverse FFT) along `axis` of a batch of
  * volumes; x and y may alias.  Replaces the cupyx.scipy.fft fft/ifft/fftn
  * calls used to prepare data (joint_recon.py:63-68, pyramid.py:65). */
@@ -92,12 +105,15 @@ long long me_encode_partials_size(int n2, int ncoils, int max_cnt, int n1);
  *   Y_s = M_s F_01 (S_c u_s)      (transform_estimation.py:207-210 without the
  *   readout FFT, which is applied once to the measured data)
  * u: image per shot at u_bstride (0: one image for all shots).  If y_meas is
- * given, Y - y_meas is produced (the residual).  y (nullable) receives the
- * values; norms (nullable, needs partials) receives ||Y - y_meas||^2 per shot
- * in FP64.  max_cnt: largest shot sample count in the batch. */
+ * given, Y - y_meas is produced (the residual) unless flags bit 0
+ * (ME_ENC_STORE_RAW) asks for Y itself.  y (nullable) receives the values;
+ * norms (nullable, needs partials) receives ||Y - y_meas||^2 per shot in FP64
+ * (the objective, joint_recon.py:108-115, when y is the prediction).
+ * max_cnt: largest shot sample count in the batch. */
+#define ME_ENC_STORE_RAW 1
 int me_encode_forward(const void* u, long long u_bstride, const void* maps, int ncoils, int nbatch, const int* n,
                       const int* samp_k0, const int* samp_k1, const int* shot_off, const int* shot_cnt, int max_cnt,
-                      void* y, const void* y_meas, double* partials, double* norms, hipStream_t stream);
+                      void* y, const void* y_meas, double* partials, double* norms, int flags, hipStream_t stream);
 
 /* Number of partial images per shot written by me_encode_adjoint (1: the
  * adjoint sums all coils itself). */
@@ -156,9 +172,14 @@ int me_cg_init(const void* b, const void* Ap, void* r, void* p, const float* P, 
 
 /* One PCG update after Ap = A p (image_estimation.py:143-163): pAp, alpha,
  * x += alpha p, r -= alpha Ap, rsnew = Re<r, P r>, the tolerance test, beta,
- * p = P r + beta p.  No-op once the tolerance test has fired. */
+ * p = P r + beta p.  No-op once the tolerance test has fired.  Optional
+ * tracking (yx, yp non-null, ny values): yx += alpha * yp with the same alpha,
+ * so that yx = E T x stays current when yp = E T p (gathered encodings). */
 int me_cg_step(void* x, void* r, void* p, const void* Ap, const float* P, long long n, double tol, double* state,
-               double* partials, hipStream_t stream);
+               double* partials, void* yx, const void* yp, long long ny, hipStream_t stream);
+
+/* out = a - b (complex, n values). */
+int me_csub(void* out, const void* a, const void* b, long long n, hipStream_t stream);
 
 /* x *= mask (uint8) (joint_recon.py:100). */
 int me_mask_mul(void* x, const unsigned char* mask, long long n, hipStream_t stream);

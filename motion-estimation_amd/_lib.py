@@ -28,8 +28,10 @@ _SIGS = {
     "me_version": [],
     "me_rigid_transform": [vp, i64, i32, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp, i32, i32, dp, vp],
     "me_rigid_deriv": [vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp, i32, vp],
+    "me_rigid_deriv_chain": [vp, i64, vp, vp, vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp, i32,
+                             i32, vp],
     "me_fft_axis": [vp, i64, vp, i64, i32, vp, vp, i32, i32, vp],
-    "me_encode_forward": [vp, i64, vp, i32, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, vp],
+    "me_encode_forward": [vp, i64, vp, i32, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, vp],
     "me_encode_adjoint": [vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, i64, vp],
     "me_permute_shift": [vp, vp, vp, vp, vp, vp, vp, C.c_float, vp],
     "me_coil_ssq": [vp, i32, i64, C.c_float, C.c_float, vp, vp, vp],
@@ -38,7 +40,8 @@ _SIGS = {
     "me_scatter_hybrid": [vp, i32, vp, vp, vp, vp, vp, i32, i64, vp, vp],
     "me_dot_re": [vp, vp, vp, i64, vp, vp, vp],
     "me_cg_init": [vp, vp, vp, vp, vp, i64, vp, vp, vp],
-    "me_cg_step": [vp, vp, vp, vp, vp, i64, C.c_double, vp, vp, vp],
+    "me_cg_step": [vp, vp, vp, vp, vp, i64, C.c_double, vp, vp, vp, vp, i64, vp],
+    "me_csub": [vp, vp, vp, i64, vp],
     "me_mask_mul": [vp, vp, i64, vp],
     "me_lm_gram": [vp, vp, i64, vp, vp, i32, i32, vp, vp, vp],
     "me_lm_solve": [vp, vp, vp, i32, vp, vp, i32, vp],

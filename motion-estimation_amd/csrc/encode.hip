@@ -62,6 +62,7 @@ struct EncArgs {
   int rows_per_wg;
   int max_cnt;
   int nbatch;
+  int store_raw;  // forward: store Y itself even when meas is given (norms are still of Y - meas)
 };
 
 __device__ __forceinline__ double block_sum_f64(double v, double* sh) {
@@ -197,10 +198,10 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
       const int cc = cg * G + q;
       if (cc >= A.ncoils) continue;
       const long long idx = ybase + ((long long)x2 * A.ncoils + cc) * cnt + w;
-      float2 val = c_scale(acc[j][q], scale);
-      if (A.meas) val = c_sub(val, A.meas[idx]);
-      if (A.y) A.y[idx] = val;
-      local += (double)val.x * val.x + (double)val.y * val.y;
+      const float2 val = c_scale(acc[j][q], scale);
+      const float2 res = A.meas ? c_sub(val, A.meas[idx]) : val;
+      if (A.y) A.y[idx] = A.store_raw ? val : res;
+      local += (double)res.x * res.x + (double)res.y * res.y;
     }
   }
   if (A.partial) {
@@ -327,13 +328,24 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
         float2 a[R1];
 #pragma unroll
         for (int m1 = 0; m1 < R1; ++m1) {
-          float2 z = make_float2(0.f, 0.f);
-          for (int w = wlo[m1]; w < whi[m1]; ++w) {
-            const int kx = sk0l[w] * xr;
-            const int ti = POW2 ? (kx & mask) : mulmod(sk0l[w], xr, n0, mask, inv_n0);
-            z = c_fmac(twc[ti], ycur[w], z);
+          // two samples per step: two independent load chains in flight
+          float2 z0 = make_float2(0.f, 0.f), z1 = make_float2(0.f, 0.f);
+          int w = wlo[m1];
+          const int we = whi[m1];
+          for (; w + 1 < we; w += 2) {
+            const int ka = sk0l[w], kb = sk0l[w + 1];
+            const float2 ya = ycur[w], yb = ycur[w + 1];
+            const int ta = POW2 ? ((ka * xr) & mask) : mulmod(ka, xr, n0, mask, inv_n0);
+            const int tb = POW2 ? ((kb * xr) & mask) : mulmod(kb, xr, n0, mask, inv_n0);
+            z0 = c_fmac(twc[ta], ya, z0);
+            z1 = c_fmac(twc[tb], yb, z1);
           }
-          a[m1] = z;
+          if (w < we) {
+            const int ka = sk0l[w];
+            const int ta = POW2 ? ((ka * xr) & mask) : mulmod(ka, xr, n0, mask, inv_n0);
+            z0 = c_fmac(twc[ta], ycur[w], z0);
+          }
+          a[m1] = c_add(z0, z1);
         }
         RDFT<R1, +1>::run(a);
 #pragma unroll
@@ -544,7 +556,7 @@ long long me_encode_partials_size(int n2, int ncoils, int max_cnt, int n1) {
 
 int me_encode_forward(const void* u, long long u_bstride, const void* maps, int ncoils, int nbatch, const int* n,
                       const int* samp_k0, const int* samp_k1, const int* shot_off, const int* shot_cnt, int max_cnt,
-                      void* y, const void* y_meas, double* partials, double* norms, hipStream_t stream) {
+                      void* y, const void* y_meas, double* partials, double* norms, int flags, hipStream_t stream) {
   ME_REQUIRE(u && maps && n && samp_k0 && samp_k1 && shot_off && shot_cnt, "encode_forward: null argument");
   ME_REQUIRE(nbatch >= 1 && nbatch <= 65535 && ncoils >= 1, "encode_forward: bad batch/coil count");
   ME_REQUIRE(n[0] <= kMaxCol, "encode_forward: N0 above 512");
@@ -566,6 +578,7 @@ int me_encode_forward(const void* u, long long u_bstride, const void* maps, int 
   A.y = (float2*)y;
   A.meas = (const float2*)y_meas;
   A.partial = partials;
+  A.store_raw = (flags & 1) ? 1 : 0;
   A.ncg = (ncoils + kCoilsPerWG - 1) / kCoilsPerWG;
   const int spt = 2;
   A.nchunk = max_cnt > 0 ? (max_cnt + spt * nt - 1) / (spt * nt) : 1;

@@ -225,6 +225,50 @@ std::vector<Pass> build_deriv(int p) {
   return v;
 }
 
+// Shared-chain derivative columns.  With T_d = Tr Rx Ry Rz, the six columns
+// need the rotation chain up to different levels:
+//   c1 = Rz x,  c2 = Ry c1,  c3 = Rx c2 (no translation folded)
+//   p = 0,1,2: Tr'_p c3            (three translation passes, axis-p one differentiated)
+//   p = 3:     Tr dRx c2           (Tr_y folded into dRx's last pass, then Tr_x, Tr_z)
+//   p = 4:     Tr Rx dRy c1
+//   p = 5:     Tr Rx Ry dRz x
+// 9 + 3*3 + 5 + 8 + 11 = 42 passes for all six columns instead of 6*11.
+std::vector<Pass> build_rotation_only(int rho) {
+  std::vector<Pass> v;
+  add_rotation(v, rho, false, -1, false, false, false);
+  return v;
+}
+
+std::vector<Pass> build_deriv_from_chain(int p) {
+  std::vector<Pass> v;
+  if (p <= 2) {
+    HostPhase ty = trans(1, false), tx = trans(0, false), tz = trans(2, false);
+    if (p == 0) tx = with_dtrans(tx);
+    if (p == 1) ty = with_dtrans(ty);
+    if (p == 2) tz = with_dtrans(tz);
+    v.push_back(single(1, ty));
+    v.push_back(single(0, tx));
+    v.push_back(single(2, tz));
+    return v;
+  }
+  if (p == 5) {
+    add_rotation_deriv(v, 2, -1);
+  }
+  if (p >= 4) {
+    if (p == 4)
+      add_rotation_deriv(v, 1, -1);
+    else
+      add_rotation(v, 1, false, -1, false, false, false);
+  }
+  if (p == 3)
+    add_rotation_deriv(v, 0, 1);
+  else
+    add_rotation(v, 0, false, 1, false, false, false);
+  v.push_back(single(0, trans(0, false)));
+  v.push_back(single(2, trans(2, false)));
+  return v;
+}
+
 struct Bufs {
   const float2* x;
   long long x_bs;
@@ -346,6 +390,34 @@ int me_rigid_deriv(const void* x, long long x_bstride, void* y, long long y_bstr
   if (int rc = launch_coef(theta, theta_stride, coef_ws, nbatch, stream)) return rc;
   Bufs b{(const float2*)x, x_bstride, 1, (float2*)y, y_bstride, (float2*)work, work_bstride};
   return run_passes(build_deriv(p_idx), g, b, nbatch, coef_ws, nullptr, nullptr, false, stream);
+}
+
+int me_rigid_deriv_chain(const void* x, long long x_bstride, void* c1, void* c2, void* c3, long long c_bstride,
+                         void* y, long long y_bstride, void* work, long long work_bstride, int nbatch, const int* n,
+                         const long long* st, const double* r0, const double* r1, const double* r2,
+                         const double* theta, long long theta_stride, double* coef_ws, int p_idx, int build,
+                         hipStream_t stream) {
+  VolGeom g;
+  if (int rc = make_geom(g, n, st, r0, r1, r2)) return rc;
+  ME_REQUIRE(x && c1 && c2 && c3 && y, "rigid_deriv_chain: null buffer");
+  ME_REQUIRE(p_idx >= 0 && p_idx <= 5, "rigid_deriv_chain: p_idx must be in [0, 5]");
+  ME_REQUIRE(p_idx < 3 || work, "rigid_deriv_chain: rotation derivatives need a work buffer");
+  if (int rc = launch_coef(theta, theta_stride, coef_ws, nbatch, stream)) return rc;
+  if (build) {
+    const void* src[3] = {x, c1, c2};
+    const long long sbs[3] = {x_bstride, c_bstride, c_bstride};
+    void* dst[3] = {c1, c2, c3};
+    const int rho[3] = {2, 1, 0};
+    for (int l = 0; l < 3; ++l) {
+      Bufs b{(const float2*)src[l], sbs[l], 1, (float2*)dst[l], c_bstride, nullptr, 0};
+      if (int rc = run_passes(build_rotation_only(rho[l]), g, b, nbatch, coef_ws, nullptr, nullptr, false, stream))
+        return rc;
+    }
+  }
+  const void* from = p_idx <= 2 ? c3 : (p_idx == 3 ? c2 : (p_idx == 4 ? c1 : x));
+  const long long fbs = p_idx == 5 ? x_bstride : c_bstride;
+  Bufs b{(const float2*)from, fbs, 1, (float2*)y, y_bstride, (float2*)work, work_bstride};
+  return run_passes(build_deriv_from_chain(p_idx), g, b, nbatch, coef_ws, nullptr, nullptr, false, stream);
 }
 
 int me_fft_axis(const void* x, long long x_bstride, void* y, long long y_bstride, int nbatch, const int* n,

@@ -146,6 +146,28 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_update(float2* __restrict__ 
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// y += alpha * yp with the same alpha as the x update (skipped once done):
+// keeps E T x up to date as x moves along p, from E T p of the normal operator.
+__global__ void __launch_bounds__(kVecThreads) k_cg_track(float2* __restrict__ y, const float2* __restrict__ yp,
+                                                          long long n, const double* __restrict__ st) {
+  if (st[CG_DONE] != 0.0) return;
+  const float alpha = (float)(st[CG_RSOLD] / st[CG_PAP]);
+  for (long long i = (long long)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kVecThreads) {
+    const float2 a = yp[i];
+    float2 b = y[i];
+    b.x = fmaf(alpha, a.x, b.x);
+    b.y = fmaf(alpha, a.y, b.y);
+    y[i] = b;
+  }
+}
+
+// out = a - b
+__global__ void k_csub(float2* __restrict__ out, const float2* __restrict__ a, const float2* __restrict__ b,
+                       long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = c_sub(a[i], b[i]);
+}
+
 // p = P r + beta p
 __global__ void __launch_bounds__(kVecThreads) k_cg_pupdate(float2* __restrict__ p, const float2* __restrict__ r,
                                                             const float* __restrict__ P, long long n,
@@ -335,16 +357,28 @@ int me_cg_init(const void* b, const void* Ap, void* r, void* p, const float* P, 
 }
 
 int me_cg_step(void* x, void* r, void* p, const void* Ap, const float* P, long long n, double tol, double* state,
-               double* partials, hipStream_t stream) {
+               double* partials, void* yx, const void* yp, long long ny, hipStream_t stream) {
   ME_REQUIRE(x && r && p && Ap && state && partials, "cg_step: null argument");
+  ME_REQUIRE(!yx == !yp, "cg_step: tracking needs both yx and yp");
   hipLaunchKernelGGL(k_dot_re, dim3(kVecBlocks), dim3(kVecThreads), 0, stream, (const float2*)p, (const float2*)Ap,
                      (const float*)nullptr, n, partials);
   hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(kVecThreads), 0, stream, partials, kVecBlocks, state, 1, 0.0);
   hipLaunchKernelGGL(k_cg_update, dim3(kVecBlocks), dim3(kVecThreads), 0, stream, (float2*)x, (float2*)r,
                      (const float2*)p, (const float2*)Ap, P, n, state, partials);
+  if (yx)
+    hipLaunchKernelGGL(k_cg_track, dim3(kVecBlocks), dim3(kVecThreads), 0, stream, (float2*)yx, (const float2*)yp, ny,
+                       (const double*)state);
   hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(kVecThreads), 0, stream, partials, kVecBlocks, state, 2, tol);
   hipLaunchKernelGGL(k_cg_pupdate, dim3(kVecBlocks), dim3(kVecThreads), 0, stream, (float2*)p, (const float2*)r, P, n,
                      (const double*)state);
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
+}
+
+int me_csub(void* out, const void* a, const void* b, long long n, hipStream_t stream) {
+  ME_REQUIRE(out && a && b && n >= 0, "csub: bad argument");
+  hipLaunchKernelGGL(k_csub, dim3(kVecBlocks), dim3(kVecThreads), 0, stream, (float2*)out, (const float2*)a,
+                     (const float2*)b, n);
   ME_TRY_HIP(hipGetLastError());
   return 0;
 }

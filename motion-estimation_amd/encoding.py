@@ -252,14 +252,32 @@ class SenseProblem:
                   g.n_arg(), g.st_arg(True), *g.r_args(), _lib.ptr(theta), int(theta.stride(0)), _lib.ptr(coef),
                   int(p_idx), _lib.stream(self.device))
 
-    def _encode(self, u, u_bs, s0, s1, y=None, meas=False, norms=None):
+    def _deriv_chain(self, x, out, theta, p_idx, build):
+        """Column p_idx of the LM Jacobian images from the shared rotation
+        chain c1 = Rz x, c2 = Ry c1, c3 = Rx c2 (built when ``build``).  c3
+        lives in the work buffer, so columns 0-2 come before 3-5."""
+        g = self.geom
+        B = theta.shape[0]
+        if getattr(self, "_chain", None) is None:
+            self._chain = torch.empty((2, self.batch, self.vol), dtype=torch.complex64, device=self.device)
+        W = self._wbuf()
+        coef = torch.empty((B, 16), dtype=torch.float64, device=self.device)
+        _lib.call("me_rigid_deriv_chain", _lib.ptr(x), 0, _lib.ptr(self._chain[0]), _lib.ptr(self._chain[1]),
+                  _lib.ptr(W), self.vol, _lib.ptr(out), self.vol, _lib.ptr(W), self.vol, int(B), g.n_arg(),
+                  g.st_arg(True), *g.r_args(), _lib.ptr(theta), int(theta.stride(0)), _lib.ptr(coef), int(p_idx),
+                  int(bool(build)), _lib.stream(self.device))
+
+    def _encode(self, u, u_bs, s0, s1, y=None, meas=False, norms=None, store_raw=False):
+        """Gathered encodings of shots s0..s1 into y (nullable); with ``meas``
+        the residual against the measured samples (or, with ``store_raw``, the
+        encodings themselves while ``norms`` still gets the residual norms)."""
         pl = self.plan
         maxc = int(pl.cnt_h[s0:s1].max()) if s1 > s0 else 0
         _lib.call("me_encode_forward", _lib.ptr(u), u_bs, _lib.ptr(self.maps), self.ncoils, s1 - s0,
                   _lib.int3(self.shape), _lib.ptr(pl.sk0), _lib.ptr(pl.sk1), _lib.ptr(pl.off[s0:s1]),
                   _lib.ptr(pl.cnt[s0:s1]), maxc, _lib.ptr(y), _lib.ptr(self.ymeas) if meas else None,
                   _lib.ptr(self._enc_partials) if norms is not None else None, _lib.ptr(norms),
-                  _lib.stream(self.device))
+                  1 if store_raw else 0, _lib.stream(self.device))
 
     def _decode(self, y, s0, s1, out):
         pl = self.plan
@@ -284,14 +302,21 @@ class SenseProblem:
         self._rigid(parts, self.vol * self.ncg, out, theta[s0:s1], 1, flags=flags, work=self._wbuf(), mask=self.M,
                     x_parts=self.parts)
 
-    def normal(self, p, out, theta=None):
-        """out = M * sum_s T_s^H E_s^H E_s T_s p  (image_estimation.py:106-129)."""
+    def normal(self, p, out, theta=None, y=None, norms=None):
+        """out = M * sum_s T_s^H E_s^H E_s T_s p  (image_estimation.py:106-129).
+
+        The gathered encodings E_s T_s p are left in ``y`` (default: the
+        problem's sample buffer); with ``norms`` (nshots FP64) the per-shot
+        ||E_s T_s p - y_s||^2 come out of the same encoding."""
         theta = self.theta if theta is None else theta
         U = self._ubuf()
-        Y = self._ybuf()
+        Y = self._ybuf() if y is None else y
         for s0, s1 in self.batches():
             self._rigid(p, 0, U, theta[s0:s1], 0)
-            self._encode(U, self.vol, s0, s1, y=Y)
+            if norms is None:
+                self._encode(U, self.vol, s0, s1, y=Y)
+            else:
+                self._encode(U, self.vol, s0, s1, y=Y, meas=True, norms=norms[s0:s1], store_raw=True)
             self._decode(Y, s0, s1, U)
             self._adjoint_sum(U, s0, s1, out, theta)
         return out
@@ -316,25 +341,40 @@ class SenseProblem:
             self._adjoint_sum(src, s0, s1, out, theta)
         return out
 
-    def cg(self, x, b, max_iter, tol=1e-6, check_every=1, theta=None):
-        """PCG of image_estimation.py:130-165 (x updated in place)."""
+    def cg(self, x, b, max_iter, tol=1e-6, check_every=1, theta=None, track=None, norms0=None):
+        """PCG of image_estimation.py:130-165 (x updated in place).
+
+        ``track`` (a sample-sized buffer) receives E T x of the final x: the
+        initial normal operator writes E T x0 there and every step adds
+        alpha E T p, the encodings the normal operator of p already made.
+        ``norms0`` (nshots FP64) receives ||E_s T_s x0 - y_s||^2 of the
+        initial x from the same first encoding."""
         n = self.vol
         Ap = torch.empty_like(x)
         r = torch.empty_like(x)
         p = torch.empty_like(x)
         st = self._cg_state
         s = _lib.stream(self.device)
-        self.normal(x, Ap, theta)
+        self.normal(x, Ap, theta, y=track, norms=norms0)
         _lib.call("me_cg_init", _lib.ptr(b), _lib.ptr(Ap), _lib.ptr(r), _lib.ptr(p), _lib.ptr(self.P), n,
                   _lib.ptr(st), _lib.ptr(self._vec_partials), s)
+        Y = self._ybuf()
+        ny = Y.numel() if track is not None else 0
         for it in range(max_iter):
-            self.normal(p, Ap, theta)
+            self.normal(p, Ap, theta, y=Y)
             _lib.call("me_cg_step", _lib.ptr(x), _lib.ptr(r), _lib.ptr(p), _lib.ptr(Ap), _lib.ptr(self.P), n,
-                      float(tol), _lib.ptr(st), _lib.ptr(self._vec_partials), s)
+                      float(tol), _lib.ptr(st), _lib.ptr(self._vec_partials), _lib.ptr(track),
+                      _lib.ptr(Y) if track is not None else None, ny, s)
             if check_every and (it + 1) % check_every == 0 and it + 1 < max_iter:
                 if float(st[CG_DONE].item()) != 0.0:
                     break
         return x
+
+    def track_buffer(self):
+        """Sample-sized buffer for cg(track=...) / lm_step(tracked=...)."""
+        if getattr(self, "_R", None) is None:
+            self._R = torch.empty_like(self.ymeas)
+        return self._R
 
     def objective_per_shot(self, x, theta=None):
         theta = self.theta if theta is None else theta
@@ -349,13 +389,17 @@ class SenseProblem:
         """sum_s ||M_s (F S T_s x - y)||^2  (joint_recon.py:108-115)."""
         return float(self.objective_per_shot(x, theta).sum().item())
 
-    def lm_step(self, x, tol=1e-6, record=None):
+    def lm_step(self, x, tol=1e-6, record=None, tracked=False):
         """One LM update of every shot (transform_estimation.py:204-315).
-        Updates self.theta/damp/flags in place, returns T(mean) x."""
+        Updates self.theta/damp/flags in place, returns T(mean) x.
+
+        ``tracked``: the track buffer already holds E T x for this x and theta
+        (from cg(track=...)), so the residual is formed from it instead of a
+        new transform and encoding."""
         dev = self.device
         nY = self.ymeas.numel()
-        if getattr(self, "_R", None) is None:
-            self._R = torch.empty(nY, dtype=torch.complex64, device=dev)
+        R = self.track_buffer()
+        if getattr(self, "_J", None) is None:
             self._J = torch.empty((6, nY), dtype=torch.complex64, device=dev)
             self._stats = torch.empty((self.nshots, 28), dtype=torch.float64, device=dev)
             self._e1 = torch.empty(self.nshots, dtype=torch.float64, device=dev)
@@ -368,14 +412,17 @@ class SenseProblem:
         s = _lib.stream(dev)
         n0, n1, n2 = self.shape
         th = self.theta
+        if tracked:
+            _lib.call("me_csub", _lib.ptr(R), _lib.ptr(R), _lib.ptr(self.ymeas), nY, s)
         for s0, s1 in self.batches():
             B = s1 - s0
-            self._rigid(x, 0, U, th[s0:s1], 0)
-            self._encode(U, self.vol, s0, s1, y=self._R, meas=True)
+            if not tracked:
+                self._rigid(x, 0, U, th[s0:s1], 0)
+                self._encode(U, self.vol, s0, s1, y=R, meas=True)
             for p_idx in range(6):
-                self._deriv(x, U, th[s0:s1], p_idx, W)
+                self._deriv_chain(x, U, th[s0:s1], p_idx, p_idx == 0)
                 self._encode(U, self.vol, s0, s1, y=self._J[p_idx])
-            _lib.call("me_lm_gram", _lib.ptr(self._R), _lib.ptr(self._J), nY, _lib.ptr(self.plan.off[s0:s1]),
+            _lib.call("me_lm_gram", _lib.ptr(R), _lib.ptr(self._J), nY, _lib.ptr(self.plan.off[s0:s1]),
                       _lib.ptr(self.plan.cnt[s0:s1]), n2 * self.ncoils, B, _lib.ptr(self._lm_partials),
                       _lib.ptr(self._stats[s0:s1]), s)
             _lib.call("me_lm_solve", _lib.ptr(self._stats[s0:s1]), _lib.ptr(self.damp[s0:s1]), _lib.ptr(th[s0:s1]), 6,

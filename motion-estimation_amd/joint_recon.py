@@ -41,13 +41,21 @@ class JointRecon:
     def t(self):
         return self.problem.theta
 
-    def step(self):
-        """One outer iteration (joint_recon.py:92-100)."""
+    def step(self, prev_objective=None):
+        """One outer iteration (joint_recon.py:92-100).
+
+        The CG tracks E T x, so the first LM residual needs no new transform
+        and encoding.  ``prev_objective`` (nshots FP64 device tensor, or None)
+        receives the per-shot objective terms of the state this step starts
+        from, i.e. the previous iteration's objective (joint_recon.py:102),
+        computed from the CG's first encoding of that same state."""
         pb = self.problem
         b = pb.rhs()
-        pb.cg(self.x, b, self.max_cg_iter, 1e-6, check_every=self.cg_check_every)
-        for _ in range(self.max_nm_iter):
-            self.x = pb.lm_step(self.x)
+        track = pb.track_buffer() if self.max_nm_iter > 0 else None
+        pb.cg(self.x, b, self.max_cg_iter, 1e-6, check_every=self.cg_check_every, track=track,
+              norms0=prev_objective)
+        for i in range(self.max_nm_iter):
+            self.x = pb.lm_step(self.x, tracked=(i == 0))
             pb.mask_(self.x)
 
     def objective(self):
@@ -55,12 +63,17 @@ class JointRecon:
 
     def run(self):
         self.transform_history = []
-        self.objective_history = []
-        for _ in range(self.max_joint_iter):
-            self.step()
+        pending = []
+        for it in range(self.max_joint_iter):
+            buf = None
+            if self.compute_objective and it > 0:
+                buf = torch.empty(self.num_shots, dtype=torch.float64, device=self.problem.device)
+                pending.append(buf)
+            self.step(prev_objective=buf)
             self.transform_history.append(self.t.detach().cpu().numpy().copy())
-            if self.compute_objective:
-                self.objective_history.append(self.objective())
+        if self.compute_objective and self.max_joint_iter > 0:
+            pending.append(self.problem.objective_per_shot(self.x))
+        self.objective_history = [float(o.sum().item()) for o in pending]
         self.transform_history = np.stack(self.transform_history) if self.transform_history else np.zeros((0, self.num_shots, 6))
         self.objective_history = np.array(self.objective_history)
         return self.image(), self.t

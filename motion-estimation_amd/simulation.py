@@ -106,7 +106,7 @@ def generate_corrupted_kspace(gt_img, mps, A, transforms, batch=None):
         maxc = int(plan.cnt_h[s0:s1].max())
         _lib.call("me_encode_forward", _lib.ptr(U), vol, _lib.ptr(mp), ncoils, s1 - s0, _lib.int3(shape),
                   _lib.ptr(plan.sk0), _lib.ptr(plan.sk1), _lib.ptr(plan.off[s0:s1]), _lib.ptr(plan.cnt[s0:s1]), maxc,
-                  _lib.ptr(Y), None, None, None, s)
+                  _lib.ptr(Y), None, None, None, 0, s)
     del U
     kh = torch.zeros((ncoils, *shape), dtype=torch.complex64, device=dev)
     _lib.call("me_scatter_hybrid", _lib.ptr(Y), ncoils, _lib.int3(shape), _lib.ptr(plan.sk0), _lib.ptr(plan.sk1),

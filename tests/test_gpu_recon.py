@@ -205,3 +205,38 @@ def test_normal_op_many_coil_groups(me, ncoils):
     refb = O.cg_rhs(ksp, mps, A_u, t, okg, orkg, M)
     assert _rel(pb.image_from_plane(pb.rhs(cache=True)), refb) < TOL
     assert _rel(pb.image_from_plane(pb.rhs(cache=False)), refb) < TOL
+
+
+def test_chain_derivative_columns_match_oracle(me, prob_data):
+    """Jacobian images from the shared rotation chain (columns 0-5 in the
+    order the LM step uses them) against the oracle's derivative."""
+    d = prob_data
+    pb = _problem(me, d)
+    th = torch.as_tensor(d["t_true"], device=pb.device)
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(SHAPE) + 1j * rng.standard_normal(SHAPE)
+    xp = pb.image_to_plane(x)
+    U = torch.empty((pb.nshots, pb.vol), dtype=torch.complex64, device=pb.device)
+    for p_idx in range(6):
+        pb._deriv_chain(xp, U, th, p_idx, p_idx == 0)
+        for s in (0, 3, 7):
+            ref = O.rigid_deriv(x, d["t_true"][s], d["okg"], d["orkg"], p_idx)
+            assert _rel(pb.image_from_plane(U[s]), ref) < TOL, (p_idx, s)
+
+
+def test_cg_tracking_and_initial_norms(me, prob_data):
+    """E T x carried through CG by alpha E T p equals a fresh encoding of the
+    final x; the first encoding's residual norms equal the objective."""
+    d = prob_data
+    pb = _problem(me, d)
+    pb.theta.copy_(torch.as_tensor(d["t_true"]))
+    rng = np.random.default_rng(10)
+    x = pb.image_to_plane(0.1 * (rng.standard_normal(SHAPE) + 1j * rng.standard_normal(SHAPE)))
+    obj0 = pb.objective_per_shot(x).cpu().numpy()
+    b = pb.rhs()
+    track = pb.track_buffer()
+    norms0 = torch.empty(pb.nshots, dtype=torch.float64, device=pb.device)
+    pb.cg(x, b, 4, 1e-9, check_every=0, track=track, norms0=norms0)
+    np.testing.assert_allclose(norms0.cpu().numpy(), obj0, rtol=1e-6)
+    fresh = pb.forward(x).clone()
+    assert _rel(track, fresh.cpu().numpy()) < TOL
