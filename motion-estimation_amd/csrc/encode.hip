@@ -147,13 +147,17 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
       uv[m1] = up[R2 * m1 + t];
     }
   }
+  // twiddle index (k0 * x0) mod n0 of each sample, advanced row by row
+  int tix[SPT];
+#pragma unroll
+  for (int j = 0; j < SPT; ++j) tix[j] = 0;
   for (int x0 = 0; x0 < n0; ++x0) {
     if (t < R2) {
       float2 a[R1];
 #pragma unroll
       for (int m1 = 0; m1 < R1; ++m1) a[m1] = c_scale(c_mul(sv[m1], uv[m1]), cm);
-      if (x0 + 1 < n0) {
-        const int ro = (x0 + 1) * n1 + t;
+      {  // unconditional (the last row reloads itself), so the loads land in place
+        const int ro = min(x0 + 1, n0 - 1) * n1 + t;
 #pragma unroll
         for (int m1 = 0; m1 < R1; ++m1) {
           sv[m1] = sp[ro + R2 * m1];
@@ -179,10 +183,12 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
 #pragma unroll
     for (int j = 0; j < SPT; ++j) {
       if (mv[j]) {
-        const float2 w = twc[(mk0[j] * x0) % n0];
+        const float2 w = twc[tix[j]];
 #pragma unroll
         for (int q = 0; q < G; ++q) acc[j][q] = c_fma(w, zrow[q * NR + mk1[j]], acc[j][q]);
       }
+      tix[j] += mk0[j];
+      tix[j] -= (tix[j] >= n0) ? n0 : 0;
     }
   }
 
@@ -213,7 +219,7 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
 // (k*x) mod n for 0 <= k, x < n <= 512: a mask when n is a power of two,
 // else one float-reciprocal quotient estimate and a single correction.
 __device__ __forceinline__ int mulmod(int k, int x, int n, int mask, float inv_n) {
-  const int p = k * x;
+  const int p = (int)__umul24(k, x);  // k, x < 512: the product fits 24 bits
   if (mask >= 0) return p & mask;
   int q = (int)((float)p * inv_n);
   int r = p - q * n;
@@ -222,42 +228,70 @@ __device__ __forceinline__ int mulmod(int k, int x, int n, int mask, float inv_n
   return r;
 }
 
-// Adjoint.  One workgroup per (readout plane x2, block of 16 rows, shot);
-// each row is one line of T threads (the line-pass layout) and the workgroup
+// Wave-wide maximum of a non-negative int, returned as a uniform value.
+__device__ __forceinline__ int wave_max_uniform(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Rows per workgroup of the general adjoint (k_encode_adj_rows): 16 (a
+// wavefront holds four column phases of all rows), 8 for lines of 32 threads.
+template <int NR>
+struct AdjT {
+  static constexpr int R1 = RowT<NR>::R1, R2 = RowT<NR>::R2, T = RowT<NR>::T;
+  static constexpr int L = T >= 32 ? 8 : 16;
+  static constexpr int NT = T * L;
+  static constexpr int RS = R1 * (R2 + 1) + 1;  // exchange row stride (odd: rows on distinct banks)
+  // LDS carve (bytes): column twiddles at 0, staged sample records, exchange,
+  // row twiddles, column pointers
+  static constexpr int OFF_REC = kMaxCol * 8;
+  static constexpr int OFF_XCH = OFF_REC + (kAdjChunk + 1) * 16;
+  static constexpr int OFF_TWR = OFF_XCH + L * RS * 8;
+  static constexpr int OFF_CP = OFF_TWR + NR * 8;
+  static constexpr int BYTES = OFF_CP + (NR + 1) * 4;
+};
+
+// Adjoint for lines of more than 16 threads.  One workgroup per (readout
+// plane x2, block of L rows, shot); each row is one line of T threads (the
+// line-pass layout) and the workgroup
 // loops over the coils, accumulating the coil sum of its rows in registers:
 //   Z_c(x0, k1) = sum_{k0 in column k1} e^{+2 pi i k0 x0/N0} Y_c(k0, k1)   (scatter)
 //   z_c(x0, :)  = IFFT_1 Z_c(x0, :)                                         (row IFFT)
 //   out(x0, :) += conj(S_c(x0, :)) z_c(x0, :)                              (coil sum)
-// A coil's samples (one coil-plane of the gathered data) are staged in LDS,
-// the next coil's samples and map rows are prefetched into registers while
-// the current coil is transformed.
+// Thread (r, t) = (tid % L, tid / L): a wavefront holds all rows of 64/L
+// column phases t, so the samples it scatters for column k1 = R2*m1 + t come
+// from only 64/L columns; the per-m1 trip count is the wave's longest of
+// them (a uniform loop without divergent exits) and shorter columns read a
+// zero record.  A coil's samples are staged in LDS as 16-byte records
+// (y, k0 pre-scaled for the twiddle address); the next coil's samples and map
+// values are prefetched into registers while the current coil is transformed.
 template <int NR, bool POW2>
-__global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
-  constexpr int R1 = RowT<NR>::R1, R2 = RowT<NR>::R2, T = RowT<NR>::T, NT = RowT<NR>::NT;
-  constexpr int L = kCoilsPerWG;  // rows per workgroup
+__global__ void __launch_bounds__(AdjT<NR>::NT) k_encode_adj_rows(EncArgs A) {
+  using AT = AdjT<NR>;
+  constexpr int R1 = AT::R1, R2 = AT::R2, NT = AT::NT, L = AT::L, RS = AT::RS;
   constexpr int SPT = (kAdjChunk + NT - 1) / NT;  // staged samples per thread
-  __shared__ float2 yc[2][kAdjChunk];
-  __shared__ int sk0l[kAdjChunk];
-  __shared__ float2 xch[L * R1 * (R2 + 1)];
-  __shared__ float2 twr[NR];
-  __shared__ float2 twc[kMaxCol];
-  __shared__ int cp[NR + 1];
+  constexpr int ZS = kAdjChunk;                   // index of the zero record
+  __shared__ __align__(16) char smem[AT::BYTES];
+  float2* twc = (float2*)smem;  // at LDS offset 0: a byte index is an address
+  float4* rec = (float4*)(smem + AT::OFF_REC);
+  float2* xch = (float2*)(smem + AT::OFF_XCH);
+  float2* twr = (float2*)(smem + AT::OFF_TWR);
+  int* cp = (int*)(smem + AT::OFF_CP);
 
   const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
-  const int nrb = (n0 + L - 1) / L;
   const int b = blockIdx.x % A.nbatch;  // shot fastest (map rows shared through L2)
   const int x2 = (blockIdx.x / A.nbatch) % n2;
   const int row0 = (blockIdx.x / A.nbatch / n2) * L;
   const int tid = threadIdx.x;
-  const int t = tid % T, r = tid / T;
+  const int r = tid % L, t = tid / L;
   const int x0 = row0 + r;
   const bool rvalid = x0 < n0;
   const int xr = rvalid ? x0 : 0;
   const int cnt = A.cnt[b], off = A.off[b];
   const int C = A.ncoils;
-  const int mask = POW2 ? n0 - 1 : -1;
+  const int mask8 = 8 * n0 - 1;  // POW2: (8 k0 x) mod 8 n0 = 8 ((k0 x) mod n0)
   const float inv_n0 = 1.0f / (float)n0;
-  (void)nrb;
 
   fill_twiddles<NR, NT>(twr);
   for (int j = tid; j < n0; j += NT) {
@@ -267,13 +301,14 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
     twc[j] = make_float2(co, s);
   }
   for (int j = tid; j <= n1; j += NT) cp[j] = A.colptr[(long long)b * (n1 + 1) + j];
+  if (tid == 0) rec[ZS] = make_float4(0.f, 0.f, 0.f, 0.f);
 
   const long long plane = (long long)n0 * n1;
   const long long vol = plane * n2;
   const long long ybase = (long long)off * n2 * C + (long long)x2 * C * cnt;  // coil c at ybase + c*cnt
   const float scale = rsqrtf((float)n0 * (float)n1);
   const float2* srow = A.maps + (long long)x2 * plane + (long long)xr * n1;  // coil c at srow + c*vol
-  auto lidx = [&](int k1, int m2) { return r * (R1 * (R2 + 1)) + k1 * (R2 + 1) + m2; };
+  auto lidx = [&](int k1, int m2) { return r * RS + k1 * (R2 + 1) + m2; };
 
   float2 acc[R2];
 #pragma unroll
@@ -282,8 +317,10 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
   for (int s0 = 0; s0 < cnt; s0 += kAdjChunk) {
     const int ns = min(kAdjChunk, cnt - s0);
     __syncthreads();  // readers of the previous chunk's tables are done
-    for (int j = tid; j < ns; j += NT) sk0l[j] = A.sk0[off + s0 + j];
-    // the thread's column ranges within this chunk (column k1 = R2*m1 + t)
+    for (int j = tid; j < ns; j += NT) {
+      const int k0 = A.sk0[off + s0 + j];
+      rec[j].z = __int_as_float(POW2 ? 8 * k0 : k0);
+    }
     // prefetch coil 0's samples and map row
     float2 ypre[SPT];
 #pragma unroll
@@ -296,24 +333,31 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
 #pragma unroll
       for (int k2 = 0; k2 < R2; ++k2) sv[k2] = srow[t + R1 * k2];
     }
-    __syncthreads();  // sk0l and cp visible
-    // the thread's sample ranges within this chunk for its columns k1 = R2*m1 + t
-    // (the same for every coil)
-    int wlo[R1], whi[R1];
-    if (t < R2) {
+    __syncthreads();  // cp visible
+    // the thread's record ranges (byte offsets) within this chunk for its
+    // columns k1 = R2*m1 + t (the same for every coil) and the wave's longest
+    // column per m1
+    int lo16[R1], wct[R1], nmax[R1];
 #pragma unroll
-      for (int m1 = 0; m1 < R1; ++m1) {
+    for (int m1 = 0; m1 < R1; ++m1) {
+      int lo = 0, hi = 0;
+      if (t < R2) {
         const int k1 = R2 * m1 + t;
-        wlo[m1] = max(cp[k1], s0) - s0;
-        whi[m1] = min(cp[k1 + 1], s0 + ns) - s0;
+        lo = max(cp[k1], s0) - s0;
+        hi = max(min(cp[k1 + 1], s0 + ns) - s0, lo);
       }
+      lo16[m1] = 16 * lo;
+      wct[m1] = hi - lo;
+      nmax[m1] = wave_max_uniform(hi - lo);
     }
+    const char* recb = smem + AT::OFF_REC;
     for (int c = 0; c < C; ++c) {
-      float2* ycur = yc[c & 1];
+      // every reader of the previous coil's records passed that coil's
+      // exchange barrier, so one record buffer suffices
 #pragma unroll
       for (int j = 0; j < SPT; ++j) {
         const int w = tid + j * NT;
-        if (w < ns) ycur[w] = ypre[j];
+        if (w < ns) *(float2*)&rec[w] = ypre[j];
       }
       if (c + 1 < C) {  // next coil's samples, in flight during this coil
 #pragma unroll
@@ -326,25 +370,30 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
       // step 1: scatter this row's spectrum, length-R1 inverse DFTs
       if (t < R2) {
         float2 a[R1];
+        // z += e^{+2 pi i k0 x0/n0} y for record j of the column (the zero
+        // record past the column's end)
+        auto term = [&](float2& z, int j, int ct, int lo) {
+          const int o = j < ct ? lo + 16 * j : 16 * ZS;
+          const float4 q = *(const float4*)(recb + o);
+          const int k = __float_as_int(q.z);
+          float2 w;
+          if (POW2)
+            w = *(const float2*)(smem + (int)(__umul24(k, xr) & mask8));
+          else
+            w = twc[mulmod(k, xr, n0, -1, inv_n0)];
+          z = c_fmac(w, make_float2(q.x, q.y), z);  // twc holds e^{-2 pi i j/n0}: conjugate
+        };
 #pragma unroll
         for (int m1 = 0; m1 < R1; ++m1) {
-          // two samples per step: two independent load chains in flight
+          // two records per step: two independent load chains in flight
           float2 z0 = make_float2(0.f, 0.f), z1 = make_float2(0.f, 0.f);
-          int w = wlo[m1];
-          const int we = whi[m1];
-          for (; w + 1 < we; w += 2) {
-            const int ka = sk0l[w], kb = sk0l[w + 1];
-            const float2 ya = ycur[w], yb = ycur[w + 1];
-            const int ta = POW2 ? ((ka * xr) & mask) : mulmod(ka, xr, n0, mask, inv_n0);
-            const int tb = POW2 ? ((kb * xr) & mask) : mulmod(kb, xr, n0, mask, inv_n0);
-            z0 = c_fmac(twc[ta], ya, z0);
-            z1 = c_fmac(twc[tb], yb, z1);
+          const int lo = lo16[m1], ct = wct[m1], nm = nmax[m1];
+          int j = 0;
+          for (; j + 1 < nm; j += 2) {
+            term(z0, j, ct, lo);
+            term(z1, j + 1, ct, lo);
           }
-          if (w < we) {
-            const int ka = sk0l[w];
-            const int ta = POW2 ? ((ka * xr) & mask) : mulmod(ka, xr, n0, mask, inv_n0);
-            z0 = c_fmac(twc[ta], ycur[w], z0);
-          }
+          if (j < nm) term(z0, j, ct, lo);
           a[m1] = c_add(z0, z1);
         }
         RDFT<R1, +1>::run(a);
@@ -374,6 +423,178 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_adj(EncArgs A) {
     float2* op = A.out + (long long)b * A.out_bs + (long long)x2 * plane + (long long)x0 * n1;
 #pragma unroll
     for (int k2 = 0; k2 < R2; ++k2) op[t + R1 * k2] = c_scale(acc[k2], scale);
+  }
+}
+
+// Adjoint for lines of at most 16 threads (N1 <= 256): one workgroup per
+// (readout plane x2, block of 64 rows, shot), looping over the coils.  In the
+// scatter a wavefront is one column phase t and its 64 lanes are the rows, so
+// every lane walks the same samples (the records of columns k1 = R2*m1 + t,
+// read at wave-uniform LDS addresses with scalar loop bounds) and only the
+// twiddle e^{+2 pi i k0 x0/N0} differs per lane; the length-R1 DFTs over m1
+// are lane-local.  The exchange then hands each row to T threads (the
+// line-pass layout, consecutive x1 across lanes) for the length-R2 DFTs, the
+// conj(S_c) product and the coil sum in registers.
+template <int NR>
+struct AdjW {
+  static constexpr int R1 = RowT<NR>::R1, R2 = RowT<NR>::R2, T = RowT<NR>::T;
+  static constexpr int L = 64;
+  static constexpr int NT = T * L;
+  static constexpr int RS = R1 * (R2 + 1) + 1;  // exchange row stride (odd)
+  static constexpr int OFF_REC = kMaxCol * 8;   // column twiddles at LDS offset 0
+  static constexpr int OFF_XCH = OFF_REC + kAdjChunk * 16;
+  static constexpr int OFF_TWR = OFF_XCH + L * RS * 8;
+  static constexpr int OFF_CP = OFF_TWR + NR * 8;
+  static constexpr int BYTES = OFF_CP + (NR + 1) * 4;
+};
+
+template <int NR, bool POW2>
+__global__ void __launch_bounds__(AdjW<NR>::NT) k_encode_adj_wave(EncArgs A) {
+  using AW = AdjW<NR>;
+  constexpr int R1 = AW::R1, R2 = AW::R2, T = AW::T, NT = AW::NT, L = AW::L, RS = AW::RS;
+  constexpr int SPT = (kAdjChunk + NT - 1) / NT;  // staged samples per thread
+  __shared__ __align__(16) char smem[AW::BYTES];
+  float2* twc = (float2*)smem;
+  float4* rec = (float4*)(smem + AW::OFF_REC);
+  float2* xch = (float2*)(smem + AW::OFF_XCH);
+  float2* twr = (float2*)(smem + AW::OFF_TWR);
+  int* cp = (int*)(smem + AW::OFF_CP);
+
+  const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
+  const int b = blockIdx.x % A.nbatch;  // shot fastest (map rows shared through L2)
+  const int x2 = (blockIdx.x / A.nbatch) % n2;
+  const int row0 = (blockIdx.x / A.nbatch / n2) * L;
+  const int tid = threadIdx.x;
+  // scatter layout: lane = row, wavefront = column phase
+  const int r = tid % L, t = tid / L;
+  const int xr = (row0 + r < n0) ? row0 + r : 0;
+  // line layout: T consecutive threads per row
+  const int r2 = tid / T, t2 = tid % T;
+  const int x0b = row0 + r2;
+  const bool rvalid2 = x0b < n0;
+  const int xr2 = rvalid2 ? x0b : 0;
+  const int cnt = A.cnt[b], off = A.off[b];
+  const int C = A.ncoils;
+  const int mask8 = 8 * n0 - 1;
+  const float inv_n0 = 1.0f / (float)n0;
+
+  fill_twiddles<NR, NT>(twr);
+  for (int j = tid; j < n0; j += NT) {
+    float s, co;
+    const int jj = (2 * j > n0) ? j - n0 : j;
+    sincospif(-2.0f * (float)jj / (float)n0, &s, &co);
+    twc[j] = make_float2(co, s);
+  }
+  for (int j = tid; j <= n1; j += NT) cp[j] = A.colptr[(long long)b * (n1 + 1) + j];
+
+  const long long plane = (long long)n0 * n1;
+  const long long vol = plane * n2;
+  const long long ybase = (long long)off * n2 * C + (long long)x2 * C * cnt;  // coil c at ybase + c*cnt
+  const float scale = rsqrtf((float)n0 * (float)n1);
+  const float2* srow = A.maps + (long long)x2 * plane + (long long)xr2 * n1;  // coil c at srow + c*vol
+  const char* recb = smem + AW::OFF_REC;
+
+  float2 acc[R2];
+#pragma unroll
+  for (int k2 = 0; k2 < R2; ++k2) acc[k2] = make_float2(0.f, 0.f);
+
+  for (int s0 = 0; s0 < cnt; s0 += kAdjChunk) {
+    const int ns = min(kAdjChunk, cnt - s0);
+    __syncthreads();  // readers of the previous chunk's tables are done
+    for (int j = tid; j < ns; j += NT) {
+      const int k0 = A.sk0[off + s0 + j];
+      rec[j].z = __int_as_float(POW2 ? 8 * k0 : k0);
+    }
+    float2 ypre[SPT];
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+      const int w = tid + j * NT;
+      ypre[j] = A.y[ybase + s0 + (w < ns ? w : 0)];
+    }
+    __syncthreads();  // cp and the k0 fields visible
+    // the wavefront's record range (byte offsets) within this chunk for each
+    // of its columns k1 = R2*m1 + t: wave-uniform
+    int lo16[R1], hi16[R1];
+#pragma unroll
+    for (int m1 = 0; m1 < R1; ++m1) {
+      int lo = 0, hi = 0;
+      if (t < R2) {
+        const int k1 = R2 * m1 + t;
+        lo = max(cp[k1], s0) - s0;
+        hi = max(min(cp[k1 + 1], s0 + ns) - s0, lo);
+      }
+      lo16[m1] = __builtin_amdgcn_readfirstlane(16 * lo);
+      hi16[m1] = __builtin_amdgcn_readfirstlane(16 * hi);
+    }
+    for (int c = 0; c < C; ++c) {
+      // every reader of the previous coil's records passed that coil's
+      // exchange barrier, so one record buffer suffices
+#pragma unroll
+      for (int j = 0; j < SPT; ++j) {
+        const int w = tid + j * NT;
+        if (w < ns) *(float2*)&rec[w] = ypre[j];
+      }
+      if (c + 1 < C) {  // next coil's samples, in flight during this coil
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) {
+          const int w = tid + j * NT;
+          ypre[j] = A.y[ybase + (long long)(c + 1) * cnt + s0 + (w < ns ? w : 0)];
+        }
+      }
+      __syncthreads();
+      // step 1: scatter (lane = row), length-R1 inverse DFTs
+      if (t < R2) {
+        float2 a[R1];
+        auto term = [&](float2& z, int o) {  // z += e^{+2 pi i k0 x0/n0} y of the record at byte o
+          const float4 q = *(const float4*)(recb + o);
+          const int k = __float_as_int(q.z);
+          float2 w;
+          if (POW2)
+            w = *(const float2*)(smem + (int)(__umul24(k, xr) & mask8));
+          else
+            w = twc[mulmod(k, xr, n0, -1, inv_n0)];
+          z = c_fmac(w, make_float2(q.x, q.y), z);  // twc holds e^{-2 pi i j/n0}: conjugate
+        };
+#pragma unroll
+        for (int m1 = 0; m1 < R1; ++m1) {
+          float2 z0 = make_float2(0.f, 0.f), z1 = make_float2(0.f, 0.f);
+          int o = lo16[m1];
+          const int oe = hi16[m1];
+          for (; o + 16 < oe; o += 32) {
+            term(z0, o);
+            term(z1, o + 16);
+          }
+          if (o < oe) term(z0, o);
+          a[m1] = c_add(z0, z1);
+        }
+        RDFT<R1, +1>::run(a);
+#pragma unroll
+        for (int k1 = 1; k1 < R1; ++k1) a[k1] = c_mulc(a[k1], twr[(t * k1) % NR]);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) xch[r * RS + k1 * (R2 + 1) + t] = a[k1];
+      }
+      __syncthreads();
+      // step 2 (line layout): length-R2 inverse DFTs, times conj(S_c),
+      // accumulated; the map loads are issued first and consumed last (the
+      // exchange reads and the DFT of the 16 wavefronts cover their latency)
+      if (t2 < R1) {
+        float2 sv[R2];
+        const float2* sc = srow + (long long)c * vol;
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) sv[k2] = sc[t2 + R1 * k2];
+        float2 X[R2];
+#pragma unroll
+        for (int m2 = 0; m2 < R2; ++m2) X[m2] = xch[r2 * RS + t2 * (R2 + 1) + m2];
+        RDFT<R2, +1>::run(X);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) acc[k2] = c_add(acc[k2], c_mulc(X[k2], sv[k2]));
+      }
+    }
+  }
+  if (t2 < R1 && rvalid2) {
+    float2* op = A.out + (long long)b * A.out_bs + (long long)x2 * plane + (long long)x0b * n1;
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) op[t2 + R1 * k2] = c_scale(acc[k2], scale);
   }
 }
 
@@ -513,6 +734,9 @@ struct FwdLaunch {
     hipLaunchKernelGGL((k_encode_fwd<NR, SPT>), grid, dim3(RowT<NR>::NT), 0, s, *a);
   }
 };
+// lines of at most 16 threads use the wave-per-phase adjoint
+template <int NR>
+constexpr bool adj_wave() { return RowT<NR>::T <= 16; }
 struct AdjLaunch {
   const EncArgs* a;
   dim3 grid;
@@ -520,11 +744,24 @@ struct AdjLaunch {
   hipStream_t s;
   template <int NR>
   void operator()() {
-    if ((a->n0 & (a->n0 - 1)) == 0)
-      hipLaunchKernelGGL((k_encode_adj<NR, true>), grid, dim3(RowT<NR>::NT), shmem, s, *a);
-    else
-      hipLaunchKernelGGL((k_encode_adj<NR, false>), grid, dim3(RowT<NR>::NT), shmem, s, *a);
+    const bool pow2 = (a->n0 & (a->n0 - 1)) == 0;
+    if constexpr (adj_wave<NR>()) {
+      if (pow2)
+        hipLaunchKernelGGL((k_encode_adj_wave<NR, true>), grid, dim3(AdjW<NR>::NT), shmem, s, *a);
+      else
+        hipLaunchKernelGGL((k_encode_adj_wave<NR, false>), grid, dim3(AdjW<NR>::NT), shmem, s, *a);
+    } else {
+      if (pow2)
+        hipLaunchKernelGGL((k_encode_adj_rows<NR, true>), grid, dim3(AdjT<NR>::NT), shmem, s, *a);
+      else
+        hipLaunchKernelGGL((k_encode_adj_rows<NR, false>), grid, dim3(AdjT<NR>::NT), shmem, s, *a);
+    }
   }
+};
+struct AdjRows {
+  int* rows;
+  template <int NR>
+  void operator()() { *rows = adj_wave<NR>() ? AdjW<NR>::L : AdjT<NR>::L; }
 };
 struct RowThreads {
   int* nt;
@@ -620,7 +857,10 @@ int me_encode_adjoint(const void* y, const void* maps, int ncoils, int nbatch, c
   A.out_bs = out_bstride;
   A.ncg = 1;
   A.nbatch = nbatch;
-  const int nrb = (n[0] + kCoilsPerWG - 1) / kCoilsPerWG;
+  int rows = 0;
+  AdjRows fr{&rows};
+  dispatch_line_length(n[1], fr);
+  const int nrb = (n[0] + rows - 1) / rows;
   ME_REQUIRE((long long)n[2] * nrb * nbatch < (1LL << 31), "encode_adjoint: grid too large");
   dim3 grid((unsigned)(n[2] * nrb * nbatch), 1u);
   AdjLaunch f{&A, grid, 0, stream};
