@@ -81,6 +81,21 @@ int me_rigid_deriv_chain(const void* x, long long x_bstride, void* c1, void* c2,
                          const double* theta, long long theta_stride, double* coef_ws, int p_idx, int build,
                          hipStream_t stream);
 
+/* Jacobian columns of the apply order T = Tr Rz Ry Rx (the composition of
+ * me_rigid_transform mode 0, transform.py:267-318) for gradients through
+ * RigidTransformCudaOptimzied.apply; the reference's derivative class
+ * (me_rigid_deriv) differentiates the reverse order instead.  Columns share
+ * the rotation chain c1 = Rx x, c2 = Tr_z Ry c1 (computed when build != 0,
+ * batch stride c_bstride) and u = T x (x_bstride/u_bstride as for
+ * me_rigid_transform):
+ *   p = 0,1,2 from u (one pass), p = 3 from x, p = 4 from c1, p = 5 from c2.
+ * work: one volume per entry (rotation columns). */
+int me_rigid_deriv_apply(const void* x, long long x_bstride, const void* u, long long u_bstride, void* c1, void* c2,
+                         long long c_bstride, void* y, long long y_bstride, void* work, long long work_bstride,
+                         int nbatch, const int* n, const long long* st, const double* r0, const double* r1,
+                         const double* r2, const double* theta, long long theta_stride, double* coef_ws, int p_idx,
+                         int build, hipStream_t stream);
+
 /* Orthonormal 1-D FFT (inverse != 0: inverse FFT) along `axis` of a batch of
  * volumes; x and y may alias.  Replaces the cupyx.scipy.fft fft/ifft/fftn
  * calls used to prepare data (joint_recon.py:63-68, pyramid.py:65). */
@@ -178,7 +193,17 @@ int me_cg_init(const void* b, const void* Ap, void* r, void* p, const float* P, 
 int me_cg_step(void* x, void* r, void* p, const void* Ap, const float* P, long long n, double tol, double* state,
                double* partials, void* yx, const void* yp, long long ny, hipStream_t stream);
 
-/* out = a - b (complex, n values). */
+/* Per-entry FP64 inner products out[b * out_stride] = Re <a_b, c_b> =
+ * sum_i Re(conj(a_b[i]) c_b[i]) over n complex values of each of nbatch
+ * entries (deterministic fixed-order sums).  partials: at least
+ * me_dot_batched_partials_size(nbatch) doubles.  Used for the per-shot
+ * parameter gradients (no counterpart in the reference, which has no
+ * gradient path). */
+long long me_dot_batched_partials_size(int nbatch);
+int me_dot_re_batched(const void* a, long long a_bstride, const void* c, long long c_bstride, long long n, int nbatch,
+                      double* partials, double* out, long long out_stride, hipStream_t stream);
+
+/* out = a - b (complex, n values). *//* out = a - b (complex, n values). */
 int me_csub(void* out, const void* a, const void* b, long long n, hipStream_t stream);
 
 /* x *= mask (uint8) (joint_recon.py:100). */

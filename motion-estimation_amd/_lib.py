@@ -42,6 +42,9 @@ _SIGS = {
     "me_cg_init": [vp, vp, vp, vp, vp, i64, vp, vp, vp],
     "me_cg_step": [vp, vp, vp, vp, vp, i64, C.c_double, vp, vp, vp, vp, i64, vp],
     "me_csub": [vp, vp, vp, i64, vp],
+    "me_dot_re_batched": [vp, i64, vp, i64, i64, i32, vp, vp, i64, vp],
+    "me_rigid_deriv_apply": [vp, i64, vp, i64, vp, vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp,
+                             i32, i32, vp],
     "me_mask_mul": [vp, vp, i64, vp],
     "me_lm_gram": [vp, vp, i64, vp, vp, i32, i32, vp, vp, vp],
     "me_lm_solve": [vp, vp, vp, i32, vp, vp, i32, vp],
@@ -55,6 +58,7 @@ _SIZES = {
     "me_vec_partials_size": ([], C.c_int),
     "me_lm_partials_size": ([i32], C.c_longlong),
     "me_encode_coil_groups": ([i32], C.c_int),
+    "me_dot_batched_partials_size": ([i32], C.c_longlong),
 }
 _OPTIONAL = {}
 

@@ -269,6 +269,45 @@ std::vector<Pass> build_deriv_from_chain(int p) {
   return v;
 }
 
+// Jacobian columns of the apply order T = Tr Rz Ry Rx (transform.py:267-318),
+// for gradients through ``apply``.  With c1 = Rx x and c2 = Tr_z Ry c1 (Tr_z
+// folded into Ry's last pass, as in apply) and u = T x:
+//   p = 0,1,2: D_p u, one pass along axis p with the factor -i k_p
+//   p = 3:     Tr_y (Rz Tr_x) (Ry Tr_z) dRx x      (10 passes)
+//   p = 4:     Tr_y (Rz Tr_x) d(Tr_z Ry) c1        (7 passes)
+//   p = 5:     Tr_y d(Tr_x Rz) c2                  (4 passes)
+// so the six columns take 3 + 3 + 3 + 10 + 7 + 4 = 30 passes given u.
+std::vector<Pass> build_apply_chain(int level) {  // level 0: c1 = Rx x; 1: c2 = Tr_z Ry c1
+  std::vector<Pass> v;
+  if (level == 0)
+    add_rotation(v, 0, false, -1, false, false, false);
+  else
+    add_rotation(v, 1, false, 2, false, false, false);
+  return v;
+}
+
+std::vector<Pass> build_deriv_apply(int p) {
+  std::vector<Pass> v;
+  if (p <= 2) {
+    HostPhase d;
+    d.active = true;
+    v.push_back(single(p, with_dtrans(d)));
+    return v;
+  }
+  if (p == 3) {
+    add_rotation_deriv(v, 0, -1);
+    add_rotation(v, 1, false, 2, false, false, false);
+  } else if (p == 4) {
+    add_rotation_deriv(v, 1, 2);
+  }
+  if (p == 5)
+    add_rotation_deriv(v, 2, 0);
+  else
+    add_rotation(v, 2, false, 0, false, false, false);
+  v.push_back(single(1, trans(1, false)));
+  return v;
+}
+
 struct Bufs {
   const float2* x;
   long long x_bs;
@@ -418,6 +457,34 @@ int me_rigid_deriv_chain(const void* x, long long x_bstride, void* c1, void* c2,
   const long long fbs = p_idx == 5 ? x_bstride : c_bstride;
   Bufs b{(const float2*)from, fbs, 1, (float2*)y, y_bstride, (float2*)work, work_bstride};
   return run_passes(build_deriv_from_chain(p_idx), g, b, nbatch, coef_ws, nullptr, nullptr, false, stream);
+}
+
+int me_rigid_deriv_apply(const void* x, long long x_bstride, const void* u, long long u_bstride, void* c1, void* c2,
+                         long long c_bstride, void* y, long long y_bstride, void* work, long long work_bstride,
+                         int nbatch, const int* n, const long long* st, const double* r0, const double* r1,
+                         const double* r2, const double* theta, long long theta_stride, double* coef_ws, int p_idx,
+                         int build, hipStream_t stream) {
+  VolGeom g;
+  if (int rc = make_geom(g, n, st, r0, r1, r2)) return rc;
+  ME_REQUIRE(y, "rigid_deriv_apply: null output");
+  ME_REQUIRE(p_idx >= 0 && p_idx <= 5, "rigid_deriv_apply: p_idx must be in [0, 5]");
+  ME_REQUIRE(p_idx >= 3 || u, "rigid_deriv_apply: translation columns need u = T x");
+  ME_REQUIRE(p_idx < 3 || work, "rigid_deriv_apply: rotation columns need a work buffer");
+  ME_REQUIRE(!build || (x && c1 && c2), "rigid_deriv_apply: building the chain needs x, c1 and c2");
+  ME_REQUIRE(p_idx != 3 || x, "rigid_deriv_apply: column 3 needs x");
+  ME_REQUIRE(p_idx != 4 || c1, "rigid_deriv_apply: column 4 needs c1");
+  ME_REQUIRE(p_idx != 5 || c2, "rigid_deriv_apply: column 5 needs c2");
+  if (int rc = launch_coef(theta, theta_stride, coef_ws, nbatch, stream)) return rc;
+  if (build) {
+    Bufs b0{(const float2*)x, x_bstride, 1, (float2*)c1, c_bstride, nullptr, 0};
+    if (int rc = run_passes(build_apply_chain(0), g, b0, nbatch, coef_ws, nullptr, nullptr, false, stream)) return rc;
+    Bufs b1{(const float2*)c1, c_bstride, 1, (float2*)c2, c_bstride, nullptr, 0};
+    if (int rc = run_passes(build_apply_chain(1), g, b1, nbatch, coef_ws, nullptr, nullptr, false, stream)) return rc;
+  }
+  const void* from = p_idx <= 2 ? u : (p_idx == 3 ? x : (p_idx == 4 ? (const void*)c1 : (const void*)c2));
+  const long long fbs = p_idx <= 2 ? u_bstride : (p_idx == 3 ? x_bstride : c_bstride);
+  Bufs b{(const float2*)from, fbs, 1, (float2*)y, y_bstride, (float2*)work, work_bstride};
+  return run_passes(build_deriv_apply(p_idx), g, b, nbatch, coef_ws, nullptr, nullptr, false, stream);
 }
 
 int me_fft_axis(const void* x, long long x_bstride, void* y, long long y_bstride, int nbatch, const int* n,

@@ -21,6 +21,7 @@
 namespace me {
 
 constexpr int kVecBlocks = 1024;
+constexpr int kDotBatchBlocks = 64;  // workgroups per batch entry of the batched dot
 constexpr int kVecThreads = 256;
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -69,6 +70,29 @@ __global__ void __launch_bounds__(kVecThreads) k_finalize_sum(const double* __re
   __shared__ double sh[kVecThreads / 64];
   const double s = sum_partials(partial, nblk, sh);
   if (threadIdx.x == 0) *out = s;
+}
+
+// out[b] = Re <a_b, c_b> over n values per batch entry (grid (nblk, nbatch))
+__global__ void __launch_bounds__(kVecThreads) k_dot_re_batched(const float2* __restrict__ a, long long a_bs,
+                                                                const float2* __restrict__ c, long long c_bs,
+                                                                long long n, double* __restrict__ partial) {
+  __shared__ double sh[kVecThreads / 64];
+  const float2* ab = a + (long long)blockIdx.y * a_bs;
+  const float2* cb = c + (long long)blockIdx.y * c_bs;
+  double v = 0.0;
+  for (long long i = (long long)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kVecThreads) {
+    const float2 x = ab[i], y = cb[i];
+    v += (double)x.x * y.x + (double)x.y * y.y;
+  }
+  const double s = wg_sum<kVecThreads>(v, sh);
+  if (threadIdx.x == 0) partial[(long long)blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kVecThreads) k_finalize_batched(const double* __restrict__ partial, int nblk,
+                                                                  double* __restrict__ out, long long out_stride) {
+  __shared__ double sh[kVecThreads / 64];
+  const double s = sum_partials(partial + (long long)blockIdx.x * nblk, nblk, sh);
+  if (threadIdx.x == 0) out[(long long)blockIdx.x * out_stride] = s;
 }
 
 // ---------------------------------------------------------------- CG
@@ -371,6 +395,20 @@ int me_cg_step(void* x, void* r, void* p, const void* Ap, const float* P, long l
   hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(kVecThreads), 0, stream, partials, kVecBlocks, state, 2, tol);
   hipLaunchKernelGGL(k_cg_pupdate, dim3(kVecBlocks), dim3(kVecThreads), 0, stream, (float2*)p, (const float2*)r, P, n,
                      (const double*)state);
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
+}
+
+long long me_dot_batched_partials_size(int nbatch) { return (long long)kDotBatchBlocks * (nbatch > 0 ? nbatch : 0); }
+
+int me_dot_re_batched(const void* a, long long a_bstride, const void* c, long long c_bstride, long long n, int nbatch,
+                      double* partials, double* out, long long out_stride, hipStream_t stream) {
+  ME_REQUIRE(a && c && partials && out && n >= 0, "dot_re_batched: bad argument");
+  ME_REQUIRE(nbatch >= 1 && nbatch <= 65535, "dot_re_batched: bad batch count");
+  hipLaunchKernelGGL(k_dot_re_batched, dim3(kDotBatchBlocks, nbatch), dim3(kVecThreads), 0, stream, (const float2*)a,
+                     a_bstride, (const float2*)c, c_bstride, n, partials);
+  hipLaunchKernelGGL(k_finalize_batched, dim3(nbatch), dim3(kVecThreads), 0, stream, partials, kDotBatchBlocks, out,
+                     out_stride);
   ME_TRY_HIP(hipGetLastError());
   return 0;
 }

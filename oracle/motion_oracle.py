@@ -29,6 +29,9 @@ rigid_apply / rigid_adjoint     integer roll KAT, 90-degree permutation KAT,
                                 vs analytic, rotation-order test
 rigid_deriv                     central finite differences of the reverse-order
                                 composition Tr*Rx*Ry*Rz, out-of-range p_idx
+rigid_deriv_apply               central finite differences of rigid_apply
+dc_loss / dc_loss_grad          central finite differences of dc_loss along
+                                random image and parameter directions
 normal operator / CG            explicit matrix from basis vectors + textbook
                                 CG; closed form sum_s E_s^H E_s = I
 estimate_transform              6x6 solve vs numpy.linalg.solve on explicit
@@ -276,6 +279,64 @@ def rigid_deriv(x, params, kgrid, rkgrid, p_idx, prec="f64"):
         out = rot(out, *X)
         out = trans(out)
     return out
+
+
+def rigid_deriv_apply(x, params, kgrid, rkgrid, p_idx):
+    """d/dtheta_p of ``rigid_apply``'s T = Tr Rz Ry Rx (rotations applied x,
+    y, z: transform.py:267-318), in FP64: the exact Jacobian column of
+    RigidTransformCudaOptimzied.apply, as needed by the gradient of a loss
+    through ``apply``.  Same building blocks as the reference's derivative
+    class (translation derivative -i k_p U, transform.py:422-431; three-term
+    product rule over the shears, transform.py:433-474), in the apply order."""
+    w = WORKERS
+    q1, q2, q3, t1, t2, t3 = [float(v) for v in params]
+    a = np.array(x, dtype=np.complex128, copy=True)
+
+    def rot(a, theta, rkt, rks, at, as_, deriv):
+        vtan = np.exp(1j * np.tan(theta / 2) * rkt)
+        vsin = np.exp(-1j * np.sin(theta) * rks)
+        if not deriv:
+            return _shear(_shear(_shear(a, at, vtan, w), as_, vsin, w), at, vtan, w)
+        vtan_d = 1j * ((1 + np.tan(theta / 2) ** 2) / 2) * rkt * vtan
+        vsin_d = -1j * np.cos(theta) * rks * vsin
+        return (_shear(_shear(_shear(a, at, vtan_d, w), as_, vsin, w), at, vtan, w)
+                + _shear(_shear(_shear(a, at, vtan, w), as_, vsin_d, w), at, vtan, w)
+                + _shear(_shear(_shear(a, at, vtan, w), as_, vsin, w), at, vtan_d, w))
+
+    a = rot(a, t1, rkgrid["tan"]["x"], rkgrid["sin"]["x"], 1, 2, p_idx == 3)
+    a = rot(a, t2, rkgrid["tan"]["y"], rkgrid["sin"]["y"], 2, 0, p_idx == 4)
+    a = rot(a, t3, rkgrid["tan"]["z"], rkgrid["sin"]["z"], 0, 1, p_idx == 5)
+    U = translation_factor(kgrid, q1, q2, q3)
+    if p_idx < 3:
+        U = -1j * kgrid["xyz"[p_idx]] * U
+    return _ifftn(_fftn(a, (0, 1, 2), w) * U, (0, 1, 2), w)
+
+
+def dc_loss(x, transforms, ksp, mps, shot_mask, kgrid, rkgrid):
+    """Data-consistency loss 0.5 sum_s ||M_s (F(S T_s x) - y)||^2 in FP64 (the
+    per-shot residual of transform_estimation.py:41-43, uncentred frame, T
+    the ``apply`` of transform.py:263-318)."""
+    tot = 0.0
+    for s in range(transforms.shape[0]):
+        r = shot_residual(ksp, mps, shot_mask[s], x, transforms[s], kgrid, rkgrid)
+        tot += 0.5 * float(np.vdot(r, r).real)
+    return tot
+
+
+def dc_loss_grad(x, transforms, ksp, mps, shot_mask, kgrid, rkgrid):
+    """Gradients of ``dc_loss`` in FP64: with r_s the masked residual,
+    d/dRe(x) + i d/dIm(x) = sum_s T_s^H sum_c conj(S_c) F^H r_s, and
+    d/dtheta_{s,p} = Re <r_s, M_s F(S dT_s/dtheta_p x)>."""
+    gx = np.zeros(np.shape(x), dtype=np.complex128)
+    gt = np.zeros(np.shape(transforms), dtype=np.float64)
+    for s in range(transforms.shape[0]):
+        r = shot_residual(ksp, mps, shot_mask[s], x, transforms[s], kgrid, rkgrid)
+        gx += encode_adjoint(r, mps, shot_mask[s], transforms[s], kgrid, rkgrid)
+        for p in range(6):
+            col = rigid_deriv_apply(x, transforms[s], kgrid, rkgrid, p)
+            jc = _fftn(mps * col, (-3, -2, -1), WORKERS) * shot_mask[s]
+            gt[s, p] = float(np.vdot(r, jc).real)
+    return gx, gt
 
 
 def rigid_apply_reverse(x, params, kgrid, rkgrid, prec="f64"):

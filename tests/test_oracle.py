@@ -179,6 +179,47 @@ def test_derivative_central_difference(p_idx):
         assert _rel(d, fdf) > 1e-3
 
 
+@pytest.mark.parametrize("p_idx", range(6))
+def test_apply_order_derivative_central_difference(p_idx):
+    """rigid_deriv_apply is the Jacobian of rigid_apply (x, y, z order) and,
+    away from theta = 0, differs from the reference's reverse-order column."""
+    shape = (12, 10, 8)
+    kg, rkg = O.compute_transform_grids_voxel(shape, (1, 1, 1))
+    x = _gauss(shape, (1.0, -1.0, 0.5), 2.0) * np.exp(1j * 0.3) + 0.05 * _rand_vol(shape, 3)
+    th = np.array([0.4, -0.3, 0.2, 0.12, -0.09, 0.07])
+    d = O.rigid_deriv_apply(x, th, kg, rkg, p_idx)
+    h = 1e-4
+    e = np.zeros(6)
+    e[p_idx] = h
+    fd = (O.rigid_apply(x, th + e, kg, rkg) - O.rigid_apply(x, th - e, kg, rkg)) / (2 * h)
+    assert _rel(d, fd) < 1e-6
+    if p_idx >= 3:
+        assert _rel(d, O.rigid_deriv(x, th, kg, rkg, p_idx)) > 1e-3
+
+
+def test_dc_loss_gradient_central_difference():
+    """dc_loss_grad against central differences of dc_loss along random image
+    and parameter directions (a dropped term, a wrong conjugate or a sign
+    error in either gradient fails this)."""
+    shape = (8, 8, 4)
+    mps, mask, t, ksp = _small_problem(shape, 2, 3, seed=21, params_scale=0.1)
+    kg, rkg = O.compute_transform_grids_voxel(shape, (1, 1, 1))
+    x = _rand_vol(shape, 22)
+    gx, gt = O.dc_loss_grad(x, t, ksp, mps, mask, kg, rkg)
+    rng = np.random.default_rng(23)
+    h = 1e-5
+    for _ in range(4):
+        dx = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        dt = rng.standard_normal(t.shape)
+        fd = (O.dc_loss(x + h * dx, t + h * dt, ksp, mps, mask, kg, rkg)
+              - O.dc_loss(x - h * dx, t - h * dt, ksp, mps, mask, kg, rkg)) / (2 * h)
+        an = float(np.vdot(gx, dx).real) + float(np.sum(gt * dt))
+        assert abs(fd - an) < 1e-6 * abs(an)
+        # each part on its own
+        fdx = (O.dc_loss(x + h * dx, t, ksp, mps, mask, kg, rkg) - O.dc_loss(x - h * dx, t, ksp, mps, mask, kg, rkg)) / (2 * h)
+        assert abs(fdx - float(np.vdot(gx, dx).real)) < 1e-6 * abs(fdx)
+
+
 def test_derivative_out_of_range_returns_input():
     shape = (8, 8, 8)
     kg, rkg = O.compute_transform_grids_voxel(shape, (1, 1, 1))
