@@ -38,6 +38,10 @@ def parse():
     ap.add_argument("--config", default="c5", help="c1, c2, c3 or c5 (default: the headline workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-objective", action="store_true", help="skip the per-iteration objective")
+    ap.add_argument("--grad", action="store_true",
+                    help="differentiation workload instead of the headline: data-consistency loss forward+backward "
+                         "at c3 (192x192x128, 24 coils, 48 shots), recompute on/off, and a 64-direction central "
+                         "difference check at step 1e-3 (prints its own JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (three times), the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
@@ -155,6 +159,70 @@ def cpu_baseline(shape, ncoils, nshots, n_cg):
                         f"into one outer iteration ({nshots} shots, {n_cg} CG, 27 LM dots/shot) = {outer:.0f}s"))
 
 
+def grad_bench(args, torch, me, dev):
+    """Forward+backward of the data-consistency loss at c3 with recomputed or
+    stored sheared intermediates, plus a central-difference check."""
+    import numpy as np
+
+    data = me.simulation.make_config("c3", device=dev)
+    cfg = data["config"]
+    shape, ncoils, nshots = cfg["shape"], cfg["coils"], cfg["nshots"]
+    kg, rkg = me.compute_transform_grids_voxel(shape, (1, 1, 1))
+    pb = me.encoding.SenseProblem(data["ksp"], data["mps"], data["A"], kg, rkg, centered=True)
+    x0 = torch.fft.ifftshift(data["gt"].to(dev), dim=(0, 1, 2)).to(torch.complex64)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(5)
+    # evaluation point: the true image with the motion not yet estimated
+    # (small random parameters), where the loss is far from its minimum
+    t0 = 0.01 * torch.randn((nshots, 6), dtype=torch.float32, device=dev, generator=gen)
+    del data
+
+    def fwd_bwd(recompute):
+        x = x0.clone().requires_grad_(True)
+        th = t0.clone().requires_grad_(True)
+        me.data_consistency_loss(x, th, pb, recompute=recompute).backward()
+        return x.grad, th.grad
+
+    res = {}
+    for name, fn in (("forward", lambda: me.data_consistency_loss(x0, t0, pb)),
+                     ("fwd_bwd_recompute", lambda: fwd_bwd(True)),
+                     ("fwd_bwd_stored", lambda: fwd_bwd(False))):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t_start) * 1e3 / args.steps
+        log(f"{name}: {res[name]:.2f} ms")
+    gx, gt = fwd_bwd(True)
+    gnx, gnt = float(torch.linalg.vector_norm(gx)), float(torch.linalg.vector_norm(gt))
+    rel, scaled = [], []
+    h = 1e-3
+    with torch.no_grad():
+        for _ in range(64):
+            dx = torch.randn(shape, dtype=torch.complex64, device=dev, generator=gen)
+            dt = torch.randn((nshots, 6), dtype=torch.float32, device=dev, generator=gen)
+            fd = (float(me.data_consistency_loss(x0 + h * dx, t0 + h * dt, pb))
+                  - float(me.data_consistency_loss(x0 - h * dx, t0 - h * dt, pb))) / (2 * h)
+            an = float(torch.vdot(gx.flatten(), dx.flatten()).real) + float((gt * dt).sum())
+            rel.append(abs(fd - an) / abs(an))
+            scaled.append(abs(fd - an) / (gnx * float(torch.linalg.vector_norm(dx))
+                                          + gnt * float(torch.linalg.vector_norm(dt))))
+    out = {
+        "metric": "data-consistency loss forward+backward per second at 192x192x128, 24 coils, 48 shots",
+        "value": round(1e3 / res["fwd_bwd_recompute"], 3), "unit": "fwd+bwd/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": True, "dtype": "c64", "data": "synthetic",
+        "config": {"workload": f"c3: {shape[0]}x{shape[1]}x{shape[2]} complex64, {ncoils} coils, {nshots} "
+                               f"DISORDER shots, gradients w.r.t. image and 48x6 motion"},
+        "ms": {k: round(v, 3) for k, v in res.items()},
+        "fd_check": {"directions": 64, "step": h, "median_rel": float(np.median(rel)), "max_rel": float(max(rel)),
+                     "max_rel_to_gradient_scale": float(max(scaled))},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -172,6 +240,9 @@ def main():
     me = ge.load_package()
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.grad:
+        grad_bench(args, torch, me, dev)
+        return
     t_setup = time.perf_counter()
     log(f"generating synthetic data for {args.config}")
     data = me.simulation.make_config(args.config, device=dev)
