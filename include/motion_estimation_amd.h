@@ -223,6 +223,18 @@ int me_lm_gram(const void* r, const void* J, long long jstride, const int* shot_
 int me_lm_solve(const double* stats, const double* damp, const double* theta, int theta_stride, double* cand,
                 double* delta, int nbatch, hipStream_t stream);
 
+/* Geodesic acceleration, the reference's experimental option
+ * (transform_estimation.py:73-121).  me_lm_ga_direction: per shot,
+ * out = sum_p v_p (Je_p - J_p) / eps over the gathered samples, v = -delta,
+ * J the columns at theta and Je those at theta + eps v (same layout as
+ * me_lm_gram).  me_lm_ga_solve: with stats_dir = me_lm_gram of out against J,
+ * rhs_p = -Re<J'v, J_p>, a = (Gram + damp I)^-1 rhs, and
+ * cand = theta + v + a/2 when |a| <= |v|, else theta + v. */
+int me_lm_ga_direction(const void* J, const void* Je, long long jstride, const int* shot_off, const int* shot_cnt,
+                       int per_sample, const double* delta, double eps, void* out, int nbatch, hipStream_t stream);
+int me_lm_ga_solve(const double* stats, const double* stats_dir, const double* damp, const double* theta,
+                   int theta_stride, const double* delta, double* cand, int nbatch, hipStream_t stream);
+
 /* Accept iff e1 < e0: theta = cand, damp = max(damp/5, 1e-4), flags set if
  * |theta - cand| < tol; else damp = min(1.5 damp, 1e16)
  * (transform_estimation.py:303-309).  accepted (nullable) records decisions. */

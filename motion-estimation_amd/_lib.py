@@ -43,6 +43,8 @@ _SIGS = {
     "me_cg_step": [vp, vp, vp, vp, vp, i64, C.c_double, vp, vp, vp, vp, i64, vp],
     "me_csub": [vp, vp, vp, i64, vp],
     "me_dot_re_batched": [vp, i64, vp, i64, i64, i32, vp, vp, i64, vp],
+    "me_lm_ga_direction": [vp, vp, i64, vp, vp, i32, vp, C.c_double, vp, i32, vp],
+    "me_lm_ga_solve": [vp, vp, vp, vp, i32, vp, vp, i32, vp],
     "me_rigid_deriv_apply": [vp, i64, vp, i64, vp, vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp,
                              i32, i32, vp],
     "me_mask_mul": [vp, vp, i64, vp],

@@ -268,12 +268,9 @@ __global__ void k_lm_gram_finalize(const double* __restrict__ partial, int nblk,
 // H delta = g with H = Gram + damp*I (transform_estimation.py:225-236), then
 // candidate = theta - delta (transform_estimation.py:290).  Singular systems
 // produce non-finite deltas, as a failed LU would.
-__global__ void k_lm_solve(const double* __restrict__ stats, const double* __restrict__ damp,
-                           const double* __restrict__ theta, int th_stride, double* __restrict__ cand,
-                           double* __restrict__ delta_out, int nbatch) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbatch) return;
-  const double* st = stats + (long long)b * kStats;
+// d = (Gram + damp I)^-1 rhs for the Gram matrix packed in a stats row
+// (Gaussian elimination with partial pivoting, FP64).
+__device__ void solve6(const double* __restrict__ st, double damp, const double* rhs, double* d) {
   double H[6][7];
   int k = 7;
   for (int a = 0; a < 6; ++a)
@@ -283,8 +280,8 @@ __global__ void k_lm_solve(const double* __restrict__ stats, const double* __res
       ++k;
     }
   for (int a = 0; a < 6; ++a) {
-    H[a][a] += damp[b];
-    H[a][6] = st[1 + a];
+    H[a][a] += damp;
+    H[a][6] = rhs[a];
   }
   for (int col = 0; col < 6; ++col) {
     int piv = col;
@@ -306,16 +303,78 @@ __global__ void k_lm_solve(const double* __restrict__ stats, const double* __res
       for (int j = col; j < 7; ++j) H[row][j] -= f * H[col][j];
     }
   }
-  double d[6];
   for (int row = 5; row >= 0; --row) {
     double s = H[row][6];
     for (int j = row + 1; j < 6; ++j) s -= H[row][j] * d[j];
     d[row] = s / H[row][row];
   }
+}
+
+__global__ void k_lm_solve(const double* __restrict__ stats, const double* __restrict__ damp,
+                           const double* __restrict__ theta, int th_stride, double* __restrict__ cand,
+                           double* __restrict__ delta_out, int nbatch) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbatch) return;
+  const double* st = stats + (long long)b * kStats;
+  double d[6];
+  solve6(st, damp[b], st + 1, d);
   for (int a = 0; a < 6; ++a) {
     cand[(long long)b * 6 + a] = theta[(long long)b * th_stride + a] - d[a];
     if (delta_out) delta_out[(long long)b * 6 + a] = d[a];
   }
+}
+
+// Geodesic acceleration (transform_estimation.py:78-100): per shot,
+// out = sum_p v_p (Je_p - J_p) / eps over the shot's gathered samples, with
+// v = -delta, J the columns at theta and Je those at theta + eps v.
+__global__ void __launch_bounds__(kVecThreads) k_lm_ga_direction(const float2* __restrict__ J,
+                                                                 const float2* __restrict__ Je, long long jstride,
+                                                                 const int* __restrict__ off,
+                                                                 const int* __restrict__ cnt, int per_sample,
+                                                                 const double* __restrict__ delta, float inv_eps,
+                                                                 float2* __restrict__ out) {
+  const int b = blockIdx.y;
+  const long long base = (long long)off[b] * per_sample;
+  const long long n = (long long)cnt[b] * per_sample;
+  float v[6];
+#pragma unroll
+  for (int p = 0; p < 6; ++p) v[p] = (float)(-delta[(long long)b * 6 + p]);
+  for (long long i = (long long)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kVecThreads) {
+    float2 jev = make_float2(0.f, 0.f), jv = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+      const float2 a = Je[p * jstride + base + i], c = J[p * jstride + base + i];
+      jev.x = fmaf(v[p], a.x, jev.x);
+      jev.y = fmaf(v[p], a.y, jev.y);
+      jv.x = fmaf(v[p], c.x, jv.x);
+      jv.y = fmaf(v[p], c.y, jv.y);
+    }
+    out[base + i] = make_float2((jev.x - jv.x) * inv_eps, (jev.y - jv.y) * inv_eps);
+  }
+}
+
+// transform_estimation.py:102-117: rhs_p = -Re<J'v, J_p> (stats_dir row
+// entries 1..6 from the Gram kernel run on J'v), a = (Gram + damp I)^-1 rhs,
+// v_ga = v + a/2 when |a| <= |v| (else v), candidate = theta + v_ga.
+__global__ void k_lm_ga_solve(const double* __restrict__ stats, const double* __restrict__ stats_dir,
+                              const double* __restrict__ damp, const double* __restrict__ theta, int th_stride,
+                              const double* __restrict__ delta, double* __restrict__ cand, int nbatch) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbatch) return;
+  double rhs[6], a[6], v[6];
+  for (int p = 0; p < 6; ++p) {
+    rhs[p] = -stats_dir[(long long)b * kStats + 1 + p];
+    v[p] = -delta[(long long)b * 6 + p];
+  }
+  solve6(stats + (long long)b * kStats, damp[b], rhs, a);
+  double na = 0.0, nv = 0.0;
+  for (int p = 0; p < 6; ++p) {
+    na += a[p] * a[p];
+    nv += v[p] * v[p];
+  }
+  const bool take = sqrt(na) <= 1.0 * sqrt(nv);
+  for (int p = 0; p < 6; ++p)
+    cand[(long long)b * 6 + p] = theta[(long long)b * th_stride + p] + v[p] + (take ? 0.5 * a[p] : 0.0);
 }
 
 // transform_estimation.py:303-309
@@ -446,6 +505,26 @@ int me_lm_solve(const double* stats, const double* damp, const double* theta, in
   ME_REQUIRE(stats && damp && theta && cand && nbatch >= 1, "lm_solve: bad argument");
   hipLaunchKernelGGL(k_lm_solve, dim3((nbatch + 63) / 64), dim3(64), 0, stream, stats, damp, theta, theta_stride, cand,
                      delta, nbatch);
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
+}
+
+int me_lm_ga_direction(const void* J, const void* Je, long long jstride, const int* shot_off, const int* shot_cnt,
+                       int per_sample, const double* delta, double eps, void* out, int nbatch, hipStream_t stream) {
+  ME_REQUIRE(J && Je && shot_off && shot_cnt && delta && out && eps > 0, "lm_ga_direction: bad argument");
+  ME_REQUIRE(nbatch >= 1 && nbatch <= 65535, "lm_ga_direction: bad batch count");
+  hipLaunchKernelGGL(k_lm_ga_direction, dim3(64, nbatch), dim3(kVecThreads), 0, stream, (const float2*)J,
+                     (const float2*)Je, jstride, shot_off, shot_cnt, per_sample, delta, (float)(1.0 / eps),
+                     (float2*)out);
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
+}
+
+int me_lm_ga_solve(const double* stats, const double* stats_dir, const double* damp, const double* theta,
+                   int theta_stride, const double* delta, double* cand, int nbatch, hipStream_t stream) {
+  ME_REQUIRE(stats && stats_dir && damp && theta && delta && cand && nbatch >= 1, "lm_ga_solve: bad argument");
+  hipLaunchKernelGGL(k_lm_ga_solve, dim3((nbatch + 63) / 64), dim3(64), 0, stream, stats, stats_dir, damp, theta,
+                     theta_stride, delta, cand, nbatch);
   ME_TRY_HIP(hipGetLastError());
   return 0;
 }

@@ -389,13 +389,15 @@ class SenseProblem:
         """sum_s ||M_s (F S T_s x - y)||^2  (joint_recon.py:108-115)."""
         return float(self.objective_per_shot(x, theta).sum().item())
 
-    def lm_step(self, x, tol=1e-6, record=None, tracked=False):
+    def lm_step(self, x, tol=1e-6, record=None, tracked=False, use_ga=False):
         """One LM update of every shot (transform_estimation.py:204-315).
         Updates self.theta/damp/flags in place, returns T(mean) x.
 
         ``tracked``: the track buffer already holds E T x for this x and theta
         (from cg(track=...)), so the residual is formed from it instead of a
-        new transform and encoding."""
+        new transform and encoding.  ``use_ga``: the reference's experimental
+        geodesic acceleration (transform_estimation.py:73-121): six more
+        columns at theta + 1e-4 v per shot and a second 6x6 solve."""
         dev = self.device
         nY = self.ymeas.numel()
         R = self.track_buffer()
@@ -427,6 +429,8 @@ class SenseProblem:
                       _lib.ptr(self._stats[s0:s1]), s)
             _lib.call("me_lm_solve", _lib.ptr(self._stats[s0:s1]), _lib.ptr(self.damp[s0:s1]), _lib.ptr(th[s0:s1]), 6,
                       _lib.ptr(self._cand[s0:s1]), _lib.ptr(self._delta[s0:s1]), B, s)
+            if use_ga:
+                self._ga_candidates(x, s0, s1)
             self._rigid(x, 0, U, self._cand[s0:s1], 0)
             self._encode(U, self.vol, s0, s1, y=None, meas=True, norms=self._e1[s0:s1])
             _lib.call("me_lm_accept", _lib.ptr(self._stats[s0:s1]), _lib.ptr(self._e1[s0:s1]), _lib.ptr(th[s0:s1]), 6,
@@ -440,6 +444,34 @@ class SenseProblem:
         out = torch.empty(self.vol, dtype=torch.complex64, device=dev)
         self._rigid(x, 0, out, mean, 0)
         return out
+
+    def _ga_candidates(self, x, s0, s1, eps=1e-4):
+        """Replace the candidates of shots s0..s1 by their geodesically
+        accelerated ones (transform_estimation.py:73-121)."""
+        dev = self.device
+        nY = self.ymeas.numel()
+        B = s1 - s0
+        n2 = self.shape[2]
+        if getattr(self, "_Je", None) is None:
+            self._Je = torch.empty((6, nY), dtype=torch.complex64, device=dev)
+            self._Jpv = torch.empty(nY, dtype=torch.complex64, device=dev)
+            self._stats_dir = torch.empty((self.nshots, 28), dtype=torch.float64, device=dev)
+        s = _lib.stream(dev)
+        th_eps = self.theta[s0:s1] - eps * self._delta[s0:s1]  # theta + eps v, v = -delta
+        U = self._ubuf()
+        for p_idx in range(6):
+            self._deriv_chain(x, U, th_eps, p_idx, p_idx == 0)
+            self._encode(U, self.vol, s0, s1, y=self._Je[p_idx])
+        pl = self.plan
+        _lib.call("me_lm_ga_direction", _lib.ptr(self._J), _lib.ptr(self._Je), nY, _lib.ptr(pl.off[s0:s1]),
+                  _lib.ptr(pl.cnt[s0:s1]), n2 * self.ncoils, _lib.ptr(self._delta[s0:s1]), float(eps),
+                  _lib.ptr(self._Jpv), B, s)
+        _lib.call("me_lm_gram", _lib.ptr(self._Jpv), _lib.ptr(self._J), nY, _lib.ptr(pl.off[s0:s1]),
+                  _lib.ptr(pl.cnt[s0:s1]), n2 * self.ncoils, B, _lib.ptr(self._lm_partials),
+                  _lib.ptr(self._stats_dir[s0:s1]), s)
+        _lib.call("me_lm_ga_solve", _lib.ptr(self._stats[s0:s1]), _lib.ptr(self._stats_dir[s0:s1]),
+                  _lib.ptr(self.damp[s0:s1]), _lib.ptr(self.theta[s0:s1]), 6, _lib.ptr(self._delta[s0:s1]),
+                  _lib.ptr(self._cand[s0:s1]), B, s)
 
     def mask_(self, x):
         _lib.call("me_mask_mul", _lib.ptr(x), _lib.ptr(self.M), x.numel(), _lib.stream(self.device))

@@ -34,11 +34,11 @@ def estimate_transform(ksp, mps, shot_mask, image, kgrid, rkgrid, damp, converge
     kept only at the sampled lines.  ``max_iter`` is unused, as in the
     reference (Q9).  Shots are updated in parallel, which equals the
     reference's sequential loop because the image is fixed during it.
+    ``use_ga`` enables the reference's experimental geodesic acceleration
+    (transform_estimation.py:73-121).
     """
     if transforms is None:  # transform_estimation.py:41 indexes transforms[s]
         raise TypeError("'NoneType' object is not subscriptable: transforms is required")
-    if use_ga:
-        raise NotImplementedError("geodesic acceleration (use_ga=True) is experimental in the reference and not built")
     prob = SenseProblem(ksp, mps, shot_mask, kgrid, rkgrid, centered=False, normalize=False)
     prob.theta.copy_(as_f64(transforms, prob.device).reshape(prob.nshots, 6))
     prob.damp.copy_(as_f64(damp, prob.device).reshape(prob.nshots))
@@ -46,7 +46,7 @@ def estimate_transform(ksp, mps, shot_mask, image, kgrid, rkgrid, damp, converge
                                     else convergence_flags.cpu().numpy()).astype(np.uint8))
     prob.flags.copy_(fl.to(prob.device))
     x = prob.image_to_plane(image)
-    out = prob.lm_step(x, tol=tol)
+    out = prob.lm_step(x, tol=tol, use_ga=bool(use_ga))
     _write_back(transforms, prob.theta)
     _write_back(damp, prob.damp)
     _write_back(convergence_flags, prob.flags.bool())

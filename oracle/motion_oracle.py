@@ -37,6 +37,9 @@ normal operator / CG            explicit matrix from basis vectors + textbook
 estimate_transform              6x6 solve vs numpy.linalg.solve on explicit
                                 Jacobian; translation-only convergence KAT;
                                 Jacobian vs finite differences of the residual
+ga_direction / ga_step          closed form of J'v for translation directions
+                                (-(k.v)^2 in Fourier); gate and half-step by
+                                hand on a constructed system
 make_disorder_order             bit-exact stdlib ``random`` (the reference uses
                                 ``random.Random``); tile/union invariants
 pyramid resampling              SigPy centred fft/resize restated; crop KATs
@@ -479,20 +482,47 @@ def lm_normal_equations(residual, cols, damp_s, prec="f64"):
     return e0, g, H
 
 
+GA_EPS = 1e-4  # transform_estimation.py:76
+
+
+def ga_direction(mps, mask_s, image, params, cols, v, kgrid, rkgrid, prec="f64"):
+    """transform_estimation.py:78-100: J'v ~ (J(theta + eps v) v - J(theta) v) / eps
+    with the LM's (reverse-order) Jacobian columns."""
+    cdt = _cdt(prec)
+    jv = sum(v[p] * cols[p] for p in range(6))
+    cols_eps = shot_jacobian(mps, mask_s, image, params + GA_EPS * v, kgrid, rkgrid, prec)
+    jev = sum(v[p] * cols_eps[p] for p in range(6))
+    return ((jev - jv) / GA_EPS).astype(cdt, copy=False)
+
+
+def ga_step(H, cols, jpv, v, prec="f64"):
+    """transform_estimation.py:102-117: rhs_p = -Re<J'v, J_p>, a = H^-1 rhs,
+    v + a/2 if |a| <= |v| else v."""
+    rhs = np.array([-_vdot_real(jpv, cols[p], prec) for p in range(6)], dtype=np.float64)
+    a = np.linalg.solve(H, rhs)
+    return v + 0.5 * a if np.linalg.norm(a) <= 1.0 * np.linalg.norm(v) else v
+
+
 def estimate_transform(ksp, mps, shot_mask, image, kgrid, rkgrid, damp, convergence_flags, transforms,
-                       tol=1e-6, prec="f64", trace=None):
-    """transform_estimation.py:7-152 with use_ga=False (the default).
+                       tol=1e-6, prec="f64", trace=None, use_ga=False):
+    """transform_estimation.py:7-152.
 
     Mutates ``transforms``, ``damp`` and ``convergence_flags`` in place and
     returns T(mean) image.  ``trace``, if a list, receives per-shot
-    (e0, e1, accepted, delta)."""
+    (e0, e1, accepted, delta).  ``use_ga`` adds the reference's experimental
+    geodesic acceleration (transform_estimation.py:73-121)."""
     nshots = shot_mask.shape[0]
     for s in range(nshots):
         res = shot_residual(ksp, mps, shot_mask[s], image, transforms[s], kgrid, rkgrid, prec)
         cols = shot_jacobian(mps, shot_mask[s], image, transforms[s], kgrid, rkgrid, prec)
         e0, g, H = lm_normal_equations(res, cols, damp[s], prec)
         delta = np.linalg.solve(H, g)  # transform_estimation.py:236
-        nxt = transforms[s] - delta  # transform_estimation.py:290
+        if use_ga:
+            v = -delta
+            jpv = ga_direction(mps, shot_mask[s], image, transforms[s], cols, v, kgrid, rkgrid, prec)
+            nxt = transforms[s] + ga_step(H, cols, jpv, v, prec)
+        else:
+            nxt = transforms[s] - delta  # transform_estimation.py:290
         res1 = shot_residual(ksp, mps, shot_mask[s], image, nxt, kgrid, rkgrid, prec)
         e1 = _vdot_real(res1, res1, prec)
         acc = e1 < e0

@@ -159,6 +159,35 @@ def test_estimate_transform_matches_oracle(me, prob_data):
     assert _rel(x_got, x_ref) < TOL
 
 
+def test_estimate_transform_geodesic_acceleration_matches_oracle(me, prob_data):
+    """use_ga=True (transform_estimation.py:73-121).  J'v is a forward
+    difference at eps = 1e-4 of FP32 Jacobian columns (as in the reference's
+    own FP32 arithmetic), so it carries about 1e-3 relative rounding noise
+    that enters the half-step correction a/2; the motion tolerance for this
+    option is therefore 5e-3 (deg / voxel) and at most a tenth of the
+    acceleration's own effect, instead of the 1e-3 of the plain step."""
+    d = prob_data
+    x = O.estimate_image_cg(d["ksp_u"], d["mps_u"], d["A_u"], d["t_true"], d["okg"], d["orkg"], d["P"], d["M"],
+                            max_iter=5)
+    t0 = d["t_true"] + np.random.default_rng(8).normal(0, 0.05, d["t_true"].shape)
+    ns = t0.shape[0]
+    t_ref, damp_ref, fl_ref = t0.copy(), np.ones(ns), np.zeros(ns, dtype=bool)
+    x_ref = O.estimate_transform(d["ksp_u"], d["mps_u"], d["A_u"], x, d["okg"], d["orkg"], damp_ref, fl_ref, t_ref,
+                                 use_ga=True)
+    t_plain = t0.copy()
+    O.estimate_transform(d["ksp_u"], d["mps_u"], d["A_u"], x, d["okg"], d["orkg"], np.ones(ns),
+                         np.zeros(ns, dtype=bool), t_plain)
+    effect = _motion_err(t_plain, t_ref)  # what the acceleration changes
+    t_got, damp_got, fl_got = t0.copy(), np.ones(ns), np.zeros(ns, dtype=bool)
+    x_got = me.estimate_transform(d["ksp_u"], d["mps_u"], d["A_u"], x, d["kg"], d["rkg"], damp_got, fl_got, None, None,
+                                  None, None, transforms=t_got, use_ga=True)
+    assert np.array_equal(damp_got, damp_ref)
+    err = _motion_err(t_got, t_ref)
+    print(f"GA: effect {effect:.3e}, GPU vs oracle {err:.3e}")
+    assert err < 5e-3 and err < 0.1 * effect
+    assert _rel(x_got, x_ref) < 1e-3
+
+
 def test_estimate_transform_requires_transforms(me, prob_data):
     d = prob_data
     with pytest.raises(TypeError):

@@ -363,6 +363,40 @@ def test_lm_translation_only_converges_to_truth():
     assert np.max(np.abs(t - truth)) < 1e-4
 
 
+def test_ga_direction_translation_closed_form():
+    """For a translation-only direction v the columns' directional derivative
+    is E F^H(-(k.v)^2 F T x): the forward difference of the reference
+    (eps = 1e-4) must match it to O(eps)."""
+    shape = (8, 8, 6)
+    mps, mask, _, _ = _small_problem(shape, 2, 2, seed=31)
+    kg, rkg = O.compute_transform_grids_voxel(shape, (1, 1, 1))
+    x = _gauss(shape, (0.5, -1, 0.3), 1.5) + 0.05 * _rand_vol(shape, 32)
+    th = np.array([0.3, -0.2, 0.1, 0.05, -0.04, 0.03])
+    v = np.array([0.7, -0.4, 0.5, 0.0, 0.0, 0.0])
+    cols = O.shot_jacobian(mps, mask[0], x, th, kg, rkg)
+    jpv = O.ga_direction(mps, mask[0], x, th, cols, v, kg, rkg)
+    kv = v[0] * kg["x"] + v[1] * kg["y"] + v[2] * kg["z"]
+    # translation derivatives act after the rotations of T_d = Tr Rx Ry Rz
+    td = O.rigid_apply_reverse(x, th, kg, rkg)
+    dd = np.fft.ifftn(-(kv ** 2) * np.fft.fftn(td, norm="ortho"), norm="ortho")
+    ref = np.fft.fftn(mps * dd, axes=(-3, -2, -1), norm="ortho") * mask[0]
+    assert _rel(jpv, ref) < 1e-3
+
+
+def test_ga_step_gate_and_half_step():
+    rng = np.random.default_rng(33)
+    cols = [rng.standard_normal(20) + 1j * rng.standard_normal(20) for _ in range(6)]
+    H = np.array([[np.vdot(a, b).real for b in cols] for a in cols]) + 0.5 * np.eye(6)
+    v = rng.standard_normal(6)
+    jpv = 1e-3 * (rng.standard_normal(20) + 1j * rng.standard_normal(20))
+    rhs = np.array([-np.vdot(jpv, c).real for c in cols])
+    a = np.linalg.solve(H, rhs)
+    assert np.linalg.norm(a) <= np.linalg.norm(v)
+    np.testing.assert_allclose(O.ga_step(H, cols, jpv, v), v + 0.5 * a, rtol=1e-12)
+    big = 1e6 * jpv  # acceleration larger than the step: rejected
+    np.testing.assert_array_equal(O.ga_step(H, cols, big, v), v)
+
+
 def test_lm_damping_rules():
     """transform_estimation.py:303-309: accept -> damp/5 (floor 1e-4),
     reject -> damp*1.5 (cap 1e16)."""
