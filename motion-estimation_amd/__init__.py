@@ -21,5 +21,5 @@ from . import encoding, simulation  # noqa: F401
 from .image_estimation import estimate_image_cg  # noqa: F401
 from .transform_estimation import estimate_transform  # noqa: F401
 from .joint_recon import JointRecon  # noqa: F401
-from .pyramid import pyramid_reconstruction  # noqa: F401
+from .pyramid import pyramid_reconstruction, aggregate_sampling_masks, aggregate_transforms  # noqa: F401
 from .autograd import DataConsistencyLoss, data_consistency_loss  # noqa: F401

@@ -118,3 +118,31 @@ def pyramid_reconstruction(ksp, mps, A, nshots, n_spatial_levels=3, n_temporal_l
     img = prob.image_from_plane(x, shift="f", scale=prob.max_norm)
     pyramid_reconstruction.levels = levels
     return img.cpu().numpy(), t_est
+
+
+def aggregate_sampling_masks(A, bin_factor):
+    """pyramid.py:120-123: merge consecutive shots in bins of ``bin_factor``
+    by summing their masks (an integer count per line, as the reference's
+    ``sum(1)``).  Used only by the reference's disabled temporal levels
+    (pyramid.py:80-95); kept for API parity."""
+    nshots = A.shape[0]
+    nbins = nshots // bin_factor
+    if nbins * bin_factor != nshots:  # the reference's reshape fails too
+        raise ValueError(f"{nshots} shots do not split into bins of {bin_factor}")
+    if isinstance(A, torch.Tensor):
+        return A.reshape(nbins, bin_factor, *A.shape[1:]).sum(1)
+    A = np.asarray(A)
+    return A.reshape((nbins, bin_factor, *A.shape[1:])).sum(1)
+
+
+def aggregate_transforms(T, bin_factor):
+    """pyramid.py:125-128: per-bin SUM (not mean) of consecutive shots'
+    parameters, as the reference computes it."""
+    nshots = T.shape[0]
+    nbins = nshots // bin_factor
+    if nbins * bin_factor != nshots:
+        raise ValueError(f"{nshots} shots do not split into bins of {bin_factor}")
+    if isinstance(T, torch.Tensor):
+        return T.reshape(nbins, bin_factor, 6).sum(1)
+    T = np.asarray(T)
+    return T.reshape((nbins, bin_factor, 6)).sum(1)
