@@ -172,7 +172,14 @@ int me_scatter_hybrid(const void* y, int ncoils, const int* n, const int* samp_k
 int me_max_abs(const void* x, long long n, unsigned int* out_bits, hipStream_t stream);
 
 /* ---------------------------------------------------------------- CG / LM
- * CG state: double[8] = {rsold, rsnew, pAp, alpha, beta, done, iters, -}. */
+ * CG state: double[10] = {rsold, rsnew, pAp, alpha, beta, done, iters, rule,
+ * tol0, -}; the caller sets rule (and tol0) before me_cg_init:
+ *   rule 0: estimate_image_cg (image_estimation.py:156-157): stop after an
+ *           update when sqrt|<r, P r>| < tol;
+ *   rule 1: sigpy.alg.ConjugateGradient (sigpy.mri.app.SenseRecon,
+ *           simulation.py:462-466): stop before the first update when
+ *           sqrt(<r, P r>) <= tol0, on <p, A p> <= 0 without updating, and
+ *           after an update when sqrt(<r, P r>) <= tol. */
 
 /* Number of doubles in the partials workspace of the vector kernels. */
 int me_vec_partials_size(void);
@@ -202,6 +209,14 @@ int me_cg_step(void* x, void* r, void* p, const void* Ap, const float* P, long l
 long long me_dot_batched_partials_size(int nbatch);
 int me_dot_re_batched(const void* a, long long a_bstride, const void* c, long long c_bstride, long long n, int nbatch,
                       double* partials, double* out, long long out_stride, hipStream_t stream);
+
+/* out (+)= sum over nbatch entries (x + b*x_bstride) of n complex values,
+ * in a fixed order (accumulate != 0 adds to out). */
+int me_sum_batch(const void* x, long long x_bstride, int nbatch, long long n, void* out, int accumulate,
+                 hipStream_t stream);
+
+/* y += a x for complex vectors and a real scalar. */
+int me_axpy_real(void* y, double a, const void* x, long long n, hipStream_t stream);
 
 /* out = a - b (complex, n values). *//* out = a - b (complex, n values). */
 int me_csub(void* out, const void* a, const void* b, long long n, hipStream_t stream);

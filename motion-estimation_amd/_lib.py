@@ -45,6 +45,8 @@ _SIGS = {
     "me_dot_re_batched": [vp, i64, vp, i64, i64, i32, vp, vp, i64, vp],
     "me_lm_ga_direction": [vp, vp, i64, vp, vp, i32, vp, C.c_double, vp, i32, vp],
     "me_lm_ga_solve": [vp, vp, vp, vp, i32, vp, vp, i32, vp],
+    "me_sum_batch": [vp, i64, i32, i64, vp, i32, vp],
+    "me_axpy_real": [vp, C.c_double, vp, i64, vp],
     "me_rigid_deriv_apply": [vp, i64, vp, i64, vp, vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp,
                              i32, i32, vp],
     "me_mask_mul": [vp, vp, i64, vp],

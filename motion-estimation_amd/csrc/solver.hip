@@ -95,8 +95,35 @@ __global__ void __launch_bounds__(kVecThreads) k_finalize_batched(const double* 
   if (threadIdx.x == 0) out[(long long)blockIdx.x * out_stride] = s;
 }
 
+// out (+)= sum_b x_b over nbatch entries of n values (fixed order)
+__global__ void k_sum_batch(const float2* __restrict__ x, long long x_bs, int nbatch, long long n,
+                            float2* __restrict__ out, int accumulate) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    float2 acc = accumulate ? out[i] : make_float2(0.f, 0.f);
+    for (int b = 0; b < nbatch; ++b) acc = c_add(acc, x[(long long)b * x_bs + i]);
+    out[i] = acc;
+  }
+}
+
+// y += a x (a real)
+__global__ void k_axpy_real(float2* __restrict__ y, float a, const float2* __restrict__ x, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const float2 v = x[i];
+    float2 w = y[i];
+    w.x = fmaf(a, v.x, w.x);
+    w.y = fmaf(a, v.y, w.y);
+    y[i] = w;
+  }
+}
+
 // ---------------------------------------------------------------- CG
-enum { CG_RSOLD = 0, CG_RSNEW, CG_PAP, CG_ALPHA, CG_BETA, CG_DONE, CG_ITERS, CG_NSTATE };
+// CG state.  CG_RULE (set by the caller) selects the stopping rule:
+// 0 = estimate_image_cg (image_estimation.py:156-157: stop after an update
+//     when sqrt|<r, P r>| < tol);
+// 1 = sigpy.alg.ConjugateGradient (as run by sigpy.mri.app.SenseRecon): stop
+//     before any update when sqrt(<r, P r>) <= tol, when <p, A p> <= 0
+//     (no update), and after an update when sqrt(<r, P r>) <= tol.
+enum { CG_RSOLD = 0, CG_RSNEW, CG_PAP, CG_ALPHA, CG_BETA, CG_DONE, CG_ITERS, CG_RULE, CG_TOL, CG_NSTATE };
 
 // r = b - Ap; p = P r; partial sums of P |r|^2
 __global__ void __launch_bounds__(kVecThreads) k_cg_init(const float2* __restrict__ bv, const float2* __restrict__ Ap,
@@ -123,16 +150,18 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_finalize(const double* __res
   if (op == 2 && st[CG_DONE] != 0.0) return;
   const double s = sum_partials(partial, nblk, sh);
   if (threadIdx.x != 0) return;
+  const bool sigpy = st[CG_RULE] == 1.0;
   if (op == 0) {
     st[CG_RSOLD] = s;
-    st[CG_DONE] = 0.0;
+    st[CG_DONE] = (sigpy && sqrt(fabs(s)) <= st[CG_TOL]) ? 1.0 : 0.0;
     st[CG_ITERS] = 0.0;
   } else if (op == 1) {
     st[CG_PAP] = s;
+    if (sigpy && s <= 0.0) st[CG_DONE] = 1.0;
   } else {  // op 2: after the x/r update (image_estimation.py:154-164)
     st[CG_RSNEW] = s;
     st[CG_ITERS] += 1.0;
-    if (sqrt(fabs(s)) < tol) {
+    if (sigpy ? sqrt(fabs(s)) <= tol : sqrt(fabs(s)) < tol) {
       st[CG_DONE] = 1.0;
     } else {
       st[CG_BETA] = s / st[CG_RSOLD];
@@ -468,6 +497,23 @@ int me_dot_re_batched(const void* a, long long a_bstride, const void* c, long lo
                      a_bstride, (const float2*)c, c_bstride, n, partials);
   hipLaunchKernelGGL(k_finalize_batched, dim3(nbatch), dim3(kVecThreads), 0, stream, partials, kDotBatchBlocks, out,
                      out_stride);
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
+}
+
+int me_sum_batch(const void* x, long long x_bstride, int nbatch, long long n, void* out, int accumulate,
+                 hipStream_t stream) {
+  ME_REQUIRE(x && out && nbatch >= 1 && n >= 0, "sum_batch: bad argument");
+  hipLaunchKernelGGL(k_sum_batch, dim3(kVecBlocks), dim3(kVecThreads), 0, stream, (const float2*)x, x_bstride, nbatch,
+                     n, (float2*)out, accumulate);
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
+}
+
+int me_axpy_real(void* y, double a, const void* x, long long n, hipStream_t stream) {
+  ME_REQUIRE(y && x && n >= 0, "axpy_real: bad argument");
+  hipLaunchKernelGGL(k_axpy_real, dim3(kVecBlocks), dim3(kVecThreads), 0, stream, (float2*)y, (float)a,
+                     (const float2*)x, n);
   ME_TRY_HIP(hipGetLastError());
   return 0;
 }

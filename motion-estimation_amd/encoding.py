@@ -200,7 +200,7 @@ class SenseProblem:
         self.ncg = int(_lib.size("me_encode_coil_groups", self.ncoils))  # adjoint partial images per shot
         self.parts = min(self.ncg, 2)  # after folding, summed by the first adjoint pass
         self._vec_partials = torch.empty(_lib.size("me_vec_partials_size"), dtype=torch.float64, device=dev)
-        self._cg_state = torch.zeros(8, dtype=torch.float64, device=dev)
+        self._cg_state = torch.zeros(10, dtype=torch.float64, device=dev)  # rule 0 (estimate_image_cg)
         nps = _lib.size("me_encode_partials_size", n2, self.ncoils, self.plan.max_cnt, n1)
         self._enc_partials = torch.empty(max(1, nps * self.batch), dtype=torch.float64, device=dev)
 

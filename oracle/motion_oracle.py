@@ -43,6 +43,9 @@ ga_direction / ga_step          closed form of J'v for translation directions
 make_disorder_order             bit-exact stdlib ``random`` (the reference uses
                                 ``random.Random``); tile/union invariants
 pyramid resampling              SigPy centred fft/resize restated; crop KATs
+sense_recon                     full-sampling closed form (A^H A = I: one
+                                step); explicit-matrix solve; zero data
+                                stops before the first step
 ==============================  ===============================================
 """
 from __future__ import annotations
@@ -539,6 +542,64 @@ def estimate_transform(ksp, mps, shot_mask, image, kgrid, rkgrid, damp, converge
     out = rigid_apply(image, mean_t, kgrid, rkgrid, prec=prec)
     transforms -= mean_t
     return out
+
+
+# ---------------------------------------------------------------------------
+# uncorrected SENSE baseline: sigpy.mri.app.SenseRecon (simulation.py:462-466,
+# recon.py:153).  SigPy is not installed; its published algorithm is
+# restated: LinearLeastSquares with ConjugateGradient on
+# (A^H A + lamda I) x = A^H W^1/2 y, A = W^1/2 F_c S, F_c the centred ortho FFT.
+# ---------------------------------------------------------------------------
+def _fftc(a, axes):
+    return np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(a, axes=axes), axes=axes, norm="ortho"), axes=axes)
+
+
+def _ifftc(a, axes):
+    return np.fft.fftshift(np.fft.ifftn(np.fft.ifftshift(a, axes=axes), axes=axes, norm="ortho"), axes=axes)
+
+
+def sense_recon(y, mps, lamda=0.0, weights=None, P=None, max_iter=100, tol=0.0, x0=None, info=None):
+    """sigpy.mri.app.SenseRecon(y, mps, lamda, weights, P=..., max_iter, tol).run()
+    in FP64.  weights None -> rss(y over coils) > 0 (sigpy.mri.app
+    _estimate_weights).  The CG is sigpy.alg.ConjugateGradient: stop when
+    sqrt(<r, P r>) <= tol (also before the first step), or on <p, A p> <= 0
+    without updating; the last iteration updates x only."""
+    ax = (-3, -2, -1)
+    if weights is None:
+        weights = (np.sqrt(np.sum(np.abs(y) ** 2, axis=0)) > 0).astype(np.float64)
+    sw = np.sqrt(weights)
+
+    def AHA(v):
+        k = sw * _fftc(mps * v, ax)
+        return np.sum(np.conj(mps) * _ifftc(sw * k, ax), axis=0) + lamda * v
+
+    b = np.sum(np.conj(mps) * _ifftc(sw * (sw * y), ax), axis=0)
+    x = np.zeros(y.shape[1:], dtype=np.complex128) if x0 is None else np.array(x0, dtype=np.complex128)
+    Pm = (lambda v: v) if P is None else (lambda v: P * v)
+    r = b - AHA(x)
+    z = Pm(r)
+    p = z.copy()
+    rzold = float(np.vdot(r, z).real)
+    resid = math.sqrt(abs(rzold))
+    it = 0
+    while not (it >= max_iter or resid <= tol):
+        Ap = AHA(p)
+        pAp = float(np.vdot(p, Ap).real)
+        if pAp <= 0:
+            break
+        alpha = rzold / pAp
+        x = x + alpha * p
+        if it < max_iter - 1:
+            r = r - alpha * Ap
+            z = Pm(r)
+            rznew = float(np.vdot(r, z).real)
+            p = z + (rznew / rzold) * p
+            rzold = rznew
+        resid = math.sqrt(abs(rzold))
+        it += 1
+    if info is not None:
+        info["iterations"] = it
+    return x
 
 
 # ---------------------------------------------------------------------------

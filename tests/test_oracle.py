@@ -475,3 +475,59 @@ def test_sp_resize_and_centered_fft():
     f = O.sp_fft(d, (0, 1))
     assert np.allclose(f, 1 / 8)
     assert np.allclose(O.sp_ifft(f, (0, 1)), d)
+
+
+# ------------------------------------------------------------------ SENSE (f1)
+def _sense_problem(shape, ncoils, seed, frac=0.5):
+    rng = np.random.default_rng(seed)
+    mps = rng.standard_normal((ncoils, *shape)) + 1j * rng.standard_normal((ncoils, *shape))
+    mps /= np.sqrt(np.sum(np.abs(mps) ** 2, axis=0, keepdims=True))
+    lines = rng.random(shape[:2]) < frac
+    y = (rng.standard_normal((ncoils, *shape)) + 1j * rng.standard_normal((ncoils, *shape))) * lines[None, ..., None]
+    return mps, lines, y
+
+
+def test_sense_full_sampling_is_one_step():
+    shape = (6, 4, 4)
+    mps, _, _ = _sense_problem(shape, 3, 50)
+    rng = np.random.default_rng(51)
+    y = rng.standard_normal((3, *shape)) + 1j * rng.standard_normal((3, *shape))
+    info = {}
+    x = O.sense_recon(y, mps, max_iter=5, tol=1e-10, info=info)
+    direct = np.sum(np.conj(mps) * O._ifftc(y, (-3, -2, -1)), axis=0)
+    assert _rel(x, direct) < 1e-12
+    assert info["iterations"] == 1  # r = 0 (to rounding) after the first step
+
+
+def test_sense_matches_explicit_solve():
+    shape = (4, 4, 2)
+    mps, lines, y = _sense_problem(shape, 2, 52)
+    n = int(np.prod(shape))
+    w = np.broadcast_to(lines[..., None], shape).astype(float)
+    sw = np.sqrt(w)
+    A = np.zeros((2 * n, n), dtype=np.complex128)
+    for j in range(n):
+        e = np.zeros(n, dtype=np.complex128)
+        e[j] = 1
+        A[:, j] = (sw * O._fftc(mps * e.reshape(shape), (-3, -2, -1))).ravel()
+    lam = 0.3
+    AHA = A.conj().T @ A + lam * np.eye(n)
+    ref = np.linalg.solve(AHA, A.conj().T @ (sw * y).ravel()).reshape(shape)
+    P = 1.0 / (np.sum(np.abs(mps) ** 2, axis=0) + 0.5)
+    x = O.sense_recon(y, mps, lamda=lam, P=P, max_iter=4 * n)
+    assert _rel(x, ref) < 1e-8
+
+
+def test_sense_zero_data_stops_before_first_step():
+    shape = (4, 4, 2)
+    mps, lines, _ = _sense_problem(shape, 2, 53)
+    info = {}
+    x0 = _rand_vol(shape, 54)
+    y = np.zeros((2, *shape), dtype=np.complex128)
+    w = np.broadcast_to(lines[..., None], shape).astype(float)
+    # b = 0 and x0 = 0: resid = 0 <= tol before any step
+    x = O.sense_recon(y, mps, weights=w, max_iter=10, info=info)
+    assert info["iterations"] == 0 and np.all(x == 0)
+    # a nonzero start does iterate
+    O.sense_recon(y, mps, weights=w, max_iter=10, x0=x0, info=info)
+    assert info["iterations"] > 0
