@@ -40,35 +40,54 @@ struct LineT {
   static constexpr int NT = T * L;
 };
 
-// acc[k2] += phi(k) * X[k2] for one (output, input) pair on the k values owned
-// by a step-2 thread, k = t + R1*k2.  The phase exp(i k_a beta) is factored as
-// exp(2 pi i beta t/N) * exp(2 pi i beta R1/N)^k2 * (k >= N/2 ? exp(-2 pi i beta) : 1)
-// with each factor reduced in FP64 before the FP32 sincos.
-template <int N, int R1, int R2>
-__device__ __forceinline__ void phase_acc(const PhaseSpec& P, const double* cf, double rb, int t,
-                                          const float2 (&X)[R2], float2 (&acc)[R2]) {
-  const double c = P.ic >= 0 ? (double)P.sc * cf[P.ic] : 0.0;
-  const double d = P.id >= 0 ? (double)P.sd * cf[P.id] : 0.0;
-  const double beta = c * rb + d;  // angle = k_a * beta
-  float2 cur = cis2pi(frac_half(beta * (double)t / (double)N));
-  const float2 step = cis2pi(frac_half(beta * (double)R1 / (double)N));
-  const float2 wrap = cis2pi(frac_half(-beta));
-  float gam = 0.f;
-  if (P.deriv) {
-    const double g = P.ig >= 0 ? (double)P.sg * cf[P.ig] : 0.0;
-    gam = (float)(2.0 * kPi * (g * rb + (double)P.h));
-  }
+// acc[k2] += scale * phi(k) * X[k2] for one (output, input) pair on the k
+// values owned by a step-2 thread, k = t + R1*k2.  The phase exp(i k_a beta)
+// is factored as exp(2 pi i beta t/N) * exp(2 pi i beta R1/N)^k2, times
+// exp(-2 pi i beta) for k >= N/2 (k_a = 2 pi (k - N)/N there), with each
+// factor reduced in FP64 before the FP32 sincos.  For even R2 (N/2 = R1*R2/2
+// and t < R1) k >= N/2 exactly when k2 >= R2/2, so the wrap is applied once
+// at that compile-time position.
+template <int N, int R1, int R2, bool DERIV>
+__device__ __forceinline__ void phase_loop(float2 cur, float2 step, float2 wrap, float gam, int t,
+                                           const float2 (&X)[R2], float2 (&acc)[R2]) {
+  constexpr bool kEven = (R2 % 2) == 0;
 #pragma unroll
   for (int k2 = 0; k2 < R2; ++k2) {
     const int k = t + R1 * k2;
-    const bool w = 2 * k >= N;
-    float2 e = w ? c_mul(cur, wrap) : cur;
-    if (P.deriv) {
+    bool w;
+    float2 e;
+    if (kEven) {
+      w = k2 >= R2 / 2;
+      if (k2 == R2 / 2) cur = c_mul(cur, wrap);
+      e = cur;
+    } else {
+      w = 2 * k >= N;
+      e = w ? c_mul(cur, wrap) : cur;
+    }
+    if (DERIV) {
       const float kap = (float)(k - (w ? N : 0)) / (float)N;
       e = c_muli(c_scale(e, gam * kap));
     }
     acc[k2] = c_fma(X[k2], e, acc[k2]);
     if (k2 + 1 < R2) cur = c_mul(cur, step);
+  }
+}
+
+template <int N, int R1, int R2>
+__device__ __forceinline__ void phase_acc(const PhaseSpec& P, const double* cf, double rb, int t, float scale,
+                                          const float2 (&X)[R2], float2 (&acc)[R2]) {
+  const double c = P.ic >= 0 ? (double)P.sc * cf[P.ic] : 0.0;
+  const double d = P.id >= 0 ? (double)P.sd * cf[P.id] : 0.0;
+  const double beta = c * rb + d;  // angle = k_a * beta
+  const float2 cur = c_scale(cis2pi(frac_half(beta * (double)t / (double)N)), scale);
+  const float2 step = cis2pi(frac_half(beta * (double)R1 / (double)N));
+  const float2 wrap = cis2pi(frac_half(-beta));
+  if (P.deriv) {  // uniform: one of two unrolled loops
+    const double g = P.ig >= 0 ? (double)P.sg * cf[P.ig] : 0.0;
+    const float gam = (float)(2.0 * kPi * (g * rb + (double)P.h));
+    phase_loop<N, R1, R2, true>(cur, step, wrap, gam, t, X, acc);
+  } else {
+    phase_loop<N, R1, R2, false>(cur, step, wrap, 0.f, t, X, acc);
   }
 }
 
@@ -110,10 +129,13 @@ __global__ void __launch_bounds__((LineT<N, CONTIG>::NT)) k_linepass(LineArgs A)
   };
   const double ru = A.r_u ? A.r_u[u] : 0.0;
   const double rv = A.r_v ? A.r_v[v] : 0.0;
-  // padding lines (valid == false) read line 0 and are zeroed, so every load
-  // is unconditional and a thread's loads are all in flight together
-  const float vm = valid ? 1.f : 0.f;
-  const float inv_scale = MODE == 0 ? 1.0f / (float)N : rsqrtf((float)N);
+  // padding lines (valid == false) read line 0, so every load is
+  // unconditional and a thread's loads are all in flight together; lines are
+  // independent and a padding line's results are never stored
+  // MODE 0 folds the 1/N of the FFT pair into the phase accumulation;
+  // MODE 1/2 scale the stored values by N^-1/2
+  const float ph_scale = MODE == 0 ? 1.0f / (float)N : 1.0f;
+  const float out_scale = MODE == 0 ? 1.0f : rsqrtf((float)N);
   __syncthreads();
 
   const int b_begin = REDUCE ? 0 : (int)blockIdx.y;
@@ -135,8 +157,6 @@ __global__ void __launch_bounds__((LineT<N, CONTIG>::NT)) k_linepass(LineArgs A)
           const long long step = (long long)R1 * A.st_a;
 #pragma unroll
           for (int k2 = 0; k2 < R2; ++k2) X[i][k2] = st[k2 * step];
-#pragma unroll
-          for (int k2 = 0; k2 < R2; ++k2) X[i][k2] = c_scale(X[i][k2], vm);
         }
         continue;
       }
@@ -146,8 +166,6 @@ __global__ void __launch_bounds__((LineT<N, CONTIG>::NT)) k_linepass(LineArgs A)
         const long long step = (long long)R2 * A.st_a;
 #pragma unroll
         for (int n1 = 0; n1 < R1; ++n1) a[n1] = st[n1 * step];
-#pragma unroll
-        for (int n1 = 0; n1 < R1; ++n1) a[n1] = c_scale(a[n1], vm);
         RDFT<R1, -1>::run(a);
 #pragma unroll
         for (int k1 = 1; k1 < R1; ++k1) a[k1] = c_mul(a[k1], tw[(t * k1) % N]);
@@ -176,10 +194,13 @@ __global__ void __launch_bounds__((LineT<N, CONTIG>::NT)) k_linepass(LineArgs A)
           if (!P.active) continue;
           if (P.ic < 0 && P.id < 0 && !P.deriv) {  // unit phase
 #pragma unroll
-            for (int k2 = 0; k2 < R2; ++k2) acc[k2] = c_add(acc[k2], X[i][k2]);
+            for (int k2 = 0; k2 < R2; ++k2) {
+              acc[k2].x = fmaf(ph_scale, X[i][k2].x, acc[k2].x);
+              acc[k2].y = fmaf(ph_scale, X[i][k2].y, acc[k2].y);
+            }
           } else {
             const double rb = P.slot == 0 ? ru : (P.slot == 1 ? rv : 0.0);
-            phase_acc<N, R1, R2>(P, cf, rb, t, X[i], acc);
+            phase_acc<N, R1, R2>(P, cf, rb, t, ph_scale, X[i], acc);
           }
         }
       }
@@ -190,7 +211,7 @@ __global__ void __launch_bounds__((LineT<N, CONTIG>::NT)) k_linepass(LineArgs A)
           const long long step = (long long)R1 * A.st_a;
 #pragma unroll
           for (int k2 = 0; k2 < R2; ++k2) {
-            float2 val = c_scale(acc[k2], inv_scale);
+            float2 val = c_scale(acc[k2], out_scale);
             if (A.accumulate) val = c_add(val, dt[k2 * step]);
             dt[k2 * step] = val;
           }
@@ -219,7 +240,7 @@ __global__ void __launch_bounds__((LineT<N, CONTIG>::NT)) k_linepass(LineArgs A)
           const unsigned char* mk = A.mask ? A.mask + (long long)b * A.mask_bs + line_off + (long long)t * A.st_a : nullptr;
 #pragma unroll
           for (int n1 = 0; n1 < R1; ++n1) {
-            float2 val = c_scale(a[n1], inv_scale);
+            float2 val = MODE == 0 ? a[n1] : c_scale(a[n1], out_scale);
             if (mk && !mk[n1 * step]) val = make_float2(0.f, 0.f);
             if (A.accumulate) val = c_add(val, dt[n1 * step]);
             dt[n1 * step] = val;
@@ -235,7 +256,7 @@ __global__ void __launch_bounds__((LineT<N, CONTIG>::NT)) k_linepass(LineArgs A)
     const unsigned char* mk = A.mask ? A.mask + line_off + (long long)t * A.st_a : nullptr;
 #pragma unroll
     for (int n1 = 0; n1 < R1; ++n1) {
-      float2 val = c_scale(red[n1 < (REDUCE ? R1 : 1) ? n1 : 0], inv_scale);
+      float2 val = red[n1 < (REDUCE ? R1 : 1) ? n1 : 0];  // MODE 0: scaled in the phase step
       if (mk && !mk[n1 * step]) val = make_float2(0.f, 0.f);
       if (A.accumulate) val = c_add(val, dt[n1 * step]);
       dt[n1 * step] = val;

@@ -259,13 +259,8 @@ inline bool dispatch_line_length(int n, F&& f) {
 // Per-workgroup twiddle table exp(-2 pi i j / N), j in [0, N), filled once.
 template <int N, int NT>
 __device__ __forceinline__ void fill_twiddles(float2* tw) {
-  for (int j = threadIdx.x; j < N; j += NT) {
-    float s, c;
-    // -2j/N in [-2, 0]; reduce to [-1, 1] for accuracy
-    int jj = (2 * j > N) ? j - N : j;
-    sincospif(-2.0f * (float)jj / (float)N, &s, &c);
-    tw[j] = make_float2(c, s);
-  }
+  // exp(-2 pi i j / N) from the compile-time table (rounded from FP64)
+  for (int j = threadIdx.x; j < N; j += NT) tw[j] = make_float2(kTw<N>.c[j], -kTw<N>.s[j]);
 }
 
 }  // namespace me
