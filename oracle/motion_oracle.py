@@ -709,6 +709,31 @@ def pyramid_levels(ksp, mps, A, nshots, n_spatial_levels=3, sampling="disorder")
     return out
 
 
+def pyramid_reconstruction(ksp, mps, A, nshots, n_spatial_levels=3, n_joint_iters=100, sampling="disorder",
+                           final_cg_iter=100):
+    """pyramid.py:24-118: JointRecon on each coarse level (t carried across
+    levels unscaled, quirk Q13), then CG only at full resolution with the
+    estimated motion (pyramid.py:96-115).  Returns (fftshift(x*max_norm), t)."""
+    full = ksp.shape[1:]
+    t_est = np.zeros((nshots, 6), dtype=np.float64)
+    for f, ksp_res, mps_res, A_res, ds in pyramid_levels(ksp, mps, A, nshots, n_spatial_levels, sampling):
+        kg, rkg = compute_transform_grids_voxel(full, (1, 1, 1), ds)
+        app = JointRecon(ksp_res, mps_res, A_res, kg, rkg, t0=t_est, max_joint_iter=int(n_joint_iters * f),
+                         compute_objective=False)
+        _, t_est = app.run()
+        t_est = np.array(t_est)
+    k = np.fft.ifftshift(np.asarray(ksp), axes=(-3, -2, -1)).astype(np.complex128)
+    m = np.fft.ifftshift(np.asarray(mps), axes=(-3, -2, -1)).astype(np.complex128)
+    a = np.fft.ifftshift(np.asarray(A), axes=(-3, -2, -1)).astype(bool)
+    max_norm = float(np.max(np.abs(_ifftn(k, (-3, -2, -1), WORKERS))))
+    k /= max_norm
+    kg, rkg = compute_transform_grids_voxel(full, (1, 1, 1))
+    ssq = np.sum(np.abs(m) ** 2, axis=0)
+    x = estimate_image_cg(k, m, a, t_est, kg, rkg, 1 / (ssq + 1e-3), ssq > 0.1, x0=np.zeros(full, np.complex128),
+                          max_iter=final_cg_iter)
+    return np.fft.fftshift(x * max_norm), t_est
+
+
 # ---------------------------------------------------------------------------
 # a14-a15: DISORDER sampling and the corrupted forward model
 # (simulation.py:65-85, 192-245, 383-410)

@@ -1,0 +1,87 @@
+"""Size-independent properties at the headline size (256 x 256 x 192,
+32 coils, DISORDER tile 16 x 16, acceleration 2, 64 shots; 4 shots used),
+where the FP64 oracle is too slow to run: unitarity and adjoint identities
+of the transform, the encoding and the normal operator, and the round trip
+T^H T = I."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+SHAPE = (256, 256, 192)
+NC = 32
+NS = 4
+
+
+def _rel(a, b):
+    return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+
+@pytest.fixture(scope="module")
+def full(me):
+    dev = torch.device("cuda:0")
+    sim = me.simulation
+    A, _ = sim.disorder_sampling(SHAPE, (16, 16), 2, nshots=64)
+    A = A[:NS]
+    mps = sim.coil_maps(SHAPE, NC, seed=1, device=dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    ksp = torch.zeros((NC, *SHAPE), dtype=torch.complex64, device=dev)
+    kg, rkg = me.compute_transform_grids_voxel(SHAPE, (1, 1, 1))
+    pb = me.encoding.SenseProblem(ksp, mps, A, kg, rkg, centered=True)
+    del ksp, mps
+    th = torch.as_tensor(sim.motion_uniform(NS, rot_deg=5.0, trans_vox=5.0, seed=8), device=dev)
+    pb.theta.copy_(th)
+    x = torch.randn(pb.vol, dtype=torch.complex64, device=dev, generator=g)
+    y = torch.randn(pb.vol, dtype=torch.complex64, device=dev, generator=g)
+    return dict(pb=pb, th=th, x=x, y=y, g=g)
+
+
+def test_transform_unitary_and_round_trip(me, full):
+    pb, th, x = full["pb"], full["th"], full["x"]
+    U = torch.empty((NS, pb.vol), dtype=torch.complex64, device=x.device)
+    pb._rigid(x, 0, U, th, 0)
+    for s in range(NS):
+        assert abs(float(torch.linalg.vector_norm(U[s]) / torch.linalg.vector_norm(x)) - 1) < 1e-5
+    back = torch.empty_like(U)
+    pb._rigid(U, pb.vol, back, th, 1)  # T^H
+    for s in range(NS):
+        assert _rel(back[s], x) < 1e-5
+
+
+def test_transform_adjoint_identity(me, full):
+    pb, th, x, y = full["pb"], full["th"], full["x"], full["y"]
+    Tx = torch.empty((NS, pb.vol), dtype=torch.complex64, device=x.device)
+    THy = torch.empty_like(Tx)
+    pb._rigid(x, 0, Tx, th, 0)
+    pb._rigid(y, 0, THy, th, 1)
+    for s in range(NS):
+        lhs = complex(torch.vdot(Tx[s], y))
+        rhs = complex(torch.vdot(x, THy[s]))
+        assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+def test_encoding_adjoint_identity(me, full):
+    pb, x = full["pb"], full["x"]
+    U = torch.stack([x] * NS)
+    Y = torch.empty_like(pb.ymeas)
+    pb._encode(U, pb.vol, 0, NS, y=Y)
+    z = torch.randn(Y.shape, dtype=torch.complex64, device=x.device, generator=full["g"])
+    W = torch.empty((NS * pb.ncg, pb.vol), dtype=torch.complex64, device=x.device)
+    pb._decode(z, 0, NS, W)
+    W = W.reshape(NS, pb.ncg, pb.vol)[:, :pb.parts].sum(1)
+    lhs = complex(torch.vdot(Y, z))
+    rhs = sum(complex(torch.vdot(U[s], W[s])) for s in range(NS))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+def test_normal_operator_hermitian_on_support(me, full):
+    pb, x, y = full["pb"], full["x"], full["y"]
+    xm, ym = pb.mask_(x.clone()), pb.mask_(y.clone())
+    Ax, Ay = pb.zeros_image(), pb.zeros_image()
+    pb.normal(xm, Ax)
+    pb.normal(ym, Ay)
+    lhs = complex(torch.vdot(Ax, ym))
+    rhs = complex(torch.vdot(xm, Ay))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
+    assert float(torch.vdot(xm, Ax).real) > 0
