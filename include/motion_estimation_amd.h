@@ -267,6 +267,37 @@ int me_disorder_order(int max_line_number, int max_partition_number, int pat_lin
                       int block_par, unsigned long long seed, int* out_line, int* out_part, long long capacity,
                       long long* out_count, int* out_L, int* out_P);
 
+/* ---------------------------------------------------------------- calibration
+ * Preprocessing before the hot path: noise whitening (noise_whiten.py:5-14)
+ * and coil compression (coil_compression.py:6-46).  FP64 complex matrices are
+ * interleaved (re, im) doubles, row-major, ncoils <= 64. */
+
+/* Doubles of the partials workspace of me_coil_gram. */
+long long me_coil_gram_partials_size(int ncoils);
+
+/* gram[i][j] = scale * sum_v w(v) x_i(v) conj(x_j(v)) over n values per coil
+ * (x: ncoils x n complex64, coil-major; w nullable real weights).  The noise
+ * covariance of noise_whiten.py:7 (w = null, scale = 1/n) and the matrix P of
+ * coil_compression.py:17-27 (w = 1/(sum |S|^2 + reg), scale = 1). */
+int me_coil_gram(const void* x, int ncoils, long long n, const float* w, double scale, double* partials,
+                 double* gram, hipStream_t stream);
+
+/* W = (L^-1)^H with L the lower Cholesky factor of gram + reg I
+ * (noise_whiten.py:8-10, as the reference forms it). */
+int me_whitening_matrix(const double* gram, int ncoils, double reg, double* W, hipStream_t stream);
+
+/* Hermitian eigendecomposition A = V diag(eigvals) V^H (cyclic complex
+ * Jacobi, FP64), eigenvalues descending, eigenvectors as the columns of
+ * eigvecs (n x n).  For the Hermitian positive semidefinite P of
+ * coil_compression.py:30 this is its SVD (U = V, S = eigvals); eigenvector
+ * phases are those of the rotations, not LAPACK's. */
+int me_hermitian_eig(const double* A, int n, int max_sweeps, double* eigvals, double* eigvecs, hipStream_t stream);
+
+/* y[o] = sum_c M[o][c] x[c] for every voxel (M: cout x cin complex64,
+ * row-major; x: cin x n, y: cout x n).  Applies the whitening matrix
+ * (noise_whiten.py:11-12) and the compression matrix (coil_compression.py:38-43). */
+int me_coil_mix(const void* M, int cout, int cin, const void* x, long long n, void* y, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

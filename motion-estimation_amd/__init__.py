@@ -24,3 +24,4 @@ from .joint_recon import JointRecon  # noqa: F401
 from .pyramid import pyramid_reconstruction, aggregate_sampling_masks, aggregate_transforms  # noqa: F401
 from .autograd import DataConsistencyLoss, data_consistency_loss  # noqa: F401
 from .sense import SenseRecon, sense_recon  # noqa: F401
+from .calibration import coil_compression, noise_whiten  # noqa: F401

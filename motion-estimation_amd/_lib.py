@@ -47,6 +47,10 @@ _SIGS = {
     "me_lm_ga_solve": [vp, vp, vp, vp, i32, vp, vp, i32, vp],
     "me_sum_batch": [vp, i64, i32, i64, vp, i32, vp],
     "me_axpy_real": [vp, C.c_double, vp, i64, vp],
+    "me_coil_gram": [vp, i32, i64, vp, C.c_double, vp, vp, vp],
+    "me_whitening_matrix": [vp, i32, C.c_double, vp, vp],
+    "me_hermitian_eig": [vp, i32, i32, vp, vp, vp],
+    "me_coil_mix": [vp, i32, i32, vp, i64, vp, vp],
     "me_rigid_deriv_apply": [vp, i64, vp, i64, vp, vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp,
                              i32, i32, vp],
     "me_mask_mul": [vp, vp, i64, vp],
@@ -63,6 +67,7 @@ _SIZES = {
     "me_lm_partials_size": ([i32], C.c_longlong),
     "me_encode_coil_groups": ([i32], C.c_int),
     "me_dot_batched_partials_size": ([i32], C.c_longlong),
+    "me_coil_gram_partials_size": ([i32], C.c_longlong),
 }
 _OPTIONAL = {}
 

@@ -43,6 +43,11 @@ ga_direction / ga_step          closed form of J'v for translation directions
 make_disorder_order             bit-exact stdlib ``random`` (the reference uses
                                 ``random.Random``); tile/union invariants
 pyramid resampling              SigPy centred fft/resize restated; crop KATs
+noise_whiten                    inv(W^H) is lower triangular with a positive
+                                diagonal and squares to cov + reg I
+coil_compression                rank-2 maps: r from the index rule (1 coil at
+                                perc above the first share), A spans the
+                                dominant eigenvector of P
 sense_recon                     full-sampling closed form (A^H A = I: one
                                 step); explicit-matrix solve; zero data
                                 stops before the first step
@@ -600,6 +605,38 @@ def sense_recon(y, mps, lamda=0.0, weights=None, P=None, max_iter=100, tol=0.0, 
     if info is not None:
         info["iterations"] = it
     return x
+
+
+# ---------------------------------------------------------------------------
+# calibration preprocessing (noise_whiten.py:5-14, coil_compression.py:6-46)
+# ---------------------------------------------------------------------------
+def noise_whiten(ksp, noise, reg=1e-6):
+    """noise_whiten.py:5-14 (FP64): W = inv(chol(cov + reg I))^H applied to
+    the coil axis."""
+    C = ksp.shape[0]
+    nz = np.asarray(noise, dtype=np.complex128).reshape(C, -1)
+    cov = (nz @ nz.conj().T) / nz.shape[1] + reg * np.eye(C)
+    L = np.linalg.cholesky(cov)
+    W = np.linalg.inv(L).conj().T
+    return (W @ np.asarray(ksp, dtype=np.complex128).reshape(C, -1)).reshape(ksp.shape), W
+
+
+def coil_compression(mps, ksp=None, perc=0.95, reg=1e-6):
+    """coil_compression.py:6-46 (FP64): P = (S norm) S^H, U S V^H = svd(P),
+    r = argmax(cumsum(S)/sum(S) >= perc), A = U^H[:r].  Returns
+    (cc_mps, cc_ksp, A, S)."""
+    m = np.asarray(mps, dtype=np.complex128)
+    C = m.shape[0]
+    norm = 1 / (np.sum(m * m.conj(), axis=0).real + reg)
+    mf = m.reshape(C, -1)
+    P = (mf * norm.ravel()) @ mf.conj().T
+    U, S, _ = np.linalg.svd(P)
+    energy = np.cumsum(S) / np.sum(S)
+    r = int(np.argmax(energy >= perc))
+    A = U.conj().T[:r]
+    cc_mps = (A @ mf).reshape(r, *m.shape[1:])
+    cc_ksp = None if ksp is None else (A @ np.asarray(ksp, np.complex128).reshape(C, -1)).reshape(r, *m.shape[1:])
+    return cc_mps, cc_ksp, A, S
 
 
 # ---------------------------------------------------------------------------

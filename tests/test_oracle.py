@@ -531,3 +531,39 @@ def test_sense_zero_data_stops_before_first_step():
     # a nonzero start does iterate
     O.sense_recon(y, mps, weights=w, max_iter=10, x0=x0, info=info)
     assert info["iterations"] > 0
+
+
+# ------------------------------------------------------- calibration (f3)
+def test_noise_whiten_factor():
+    rng = np.random.default_rng(70)
+    C = 5
+    mix = rng.standard_normal((C, C)) + 1j * rng.standard_normal((C, C))
+    noise = mix @ (rng.standard_normal((C, 400)) + 1j * rng.standard_normal((C, 400)))
+    ksp = rng.standard_normal((C, 3, 4)) + 1j * rng.standard_normal((C, 3, 4))
+    kw, W = O.noise_whiten(ksp, noise, reg=1e-3)
+    Lr = np.linalg.inv(W.conj().T)
+    assert np.allclose(Lr, np.tril(Lr), atol=1e-12)
+    assert np.all(np.abs(np.diag(Lr).imag) < 1e-12) and np.all(np.diag(Lr).real > 0)
+    cov = noise @ noise.conj().T / noise.shape[1] + 1e-3 * np.eye(C)
+    assert np.allclose(Lr @ Lr.conj().T, cov, rtol=1e-10, atol=1e-10)
+    assert np.allclose(kw, np.einsum("ij,jab->iab", W, ksp))
+
+
+def test_coil_compression_index_rule_and_subspace():
+    rng = np.random.default_rng(71)
+    shape = (6, 5, 4)
+    b = rng.standard_normal((2, 4)) + 1j * rng.standard_normal((2, 4))  # 4 coils, rank 2
+    f = rng.standard_normal((2, *shape)) + 1j * rng.standard_normal((2, *shape))
+    mps = np.einsum("kc,kxyz->cxyz", b, f)
+    cc, _, A, S = O.coil_compression(mps, perc=0.999)
+    share = np.cumsum(S) / np.sum(S)
+    assert S[2] < 1e-9 * S[0]  # rank 2
+    assert share[0] < 0.999 <= share[1]
+    assert A.shape == (1, 4)  # argmax index 1: one coil kept (the reference's rule)
+    # A's row is the dominant eigenvector of P
+    m = mps.reshape(4, -1)
+    norm = 1 / (np.sum(np.abs(mps) ** 2, axis=0).ravel() + 1e-6)
+    P = (m * norm) @ m.conj().T
+    v = A[0].conj()
+    assert np.allclose(P @ v, S[0] * v, rtol=1e-9, atol=1e-9 * S[0])
+    assert np.allclose(cc[0], np.einsum("c,cxyz->xyz", A[0], mps))
