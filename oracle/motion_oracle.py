@@ -48,6 +48,10 @@ noise_whiten                    inv(W^H) is lower triangular with a positive
 coil_compression                rank-2 maps: r from the index rule (1 coil at
                                 perc above the first share), A spans the
                                 dominant eigenvector of P
+rx_espirit_3d                   Hankel structure of a single nonzero sample;
+                                maps from calibration data of known smooth
+                                maps agree with them up to the coil-0 phase;
+                                unit norm and real coil 0 inside the mask
 sense_recon                     full-sampling closed form (A^H A = I: one
                                 step); explicit-matrix solve; zero data
                                 stops before the first step
@@ -637,6 +641,68 @@ def coil_compression(mps, ksp=None, perc=0.95, reg=1e-6):
     cc_mps = (A @ mf).reshape(r, *m.shape[1:])
     cc_ksp = None if ksp is None else (A @ np.asarray(ksp, np.complex128).reshape(C, -1)).reshape(r, *m.shape[1:])
     return cc_mps, cc_ksp, A, S
+
+
+def hankel_fwd(x, kernel_size):
+    """espirit.py:9-40: (kx*ky*kz, nblocks, N1) with blocks ordered kx_start
+    fastest, each block flattened in Fortran order."""
+    Nx, Ny, Nz, N1 = x.shape
+    kx, ky, kz = kernel_size
+    nbx, nby, nbz = Nx - kx + 1, Ny - ky + 1, Nz - kz + 1
+    h = np.zeros((kx * ky * kz, nbx * nby * nbz, N1), dtype=np.complex128)
+    idx = 0
+    for z0 in range(nbz):
+        for y0 in range(nby):
+            for x0 in range(nbx):
+                h[:, idx, :] = x[x0:x0 + kx, y0:y0 + ky, z0:z0 + kz, :].reshape((kx * ky * kz, N1), order="F")
+                idx += 1
+    return h
+
+
+def fold_rx(h):
+    """espirit.py:42-45: (K, nb, N1) -> (K*N1, nb), row = k + K*c."""
+    hp = np.transpose(h, (0, 2, 1))
+    return hp.reshape((-1, hp.shape[2]), order="F")
+
+
+def rx_espirit_3d(calib, imsize, kernel_size=(5, 5, 5), eig_thresh=0.02, mask_thresh=0.99, info=None):
+    """espirit.py:47-239 in FP64: calib (NKx, NKy, NKz, NRx) -> maps (Nx, Ny, Nz, NRx)."""
+    calib = np.asarray(calib, dtype=np.complex128)
+    NRx = calib.shape[3]
+    kx, ky, kz = kernel_size
+    sens = np.zeros((*imsize, NRx), dtype=np.complex128)
+    H = fold_rx(hankel_fwd(calib, kernel_size))
+    U, S, _ = np.linalg.svd(H, full_matrices=False)
+    if eig_thresh < 1:
+        U = U[:, S > S[0] * eig_thresh] if S.size and S[0] > 0 else U[:, :0]
+    else:
+        U = U[:, :int(round(eig_thresh))]
+    if info is not None:
+        info["kept"] = U.shape[1]
+        info["singular_values"] = S
+    if U.shape[1] == 0:
+        return sens
+    nc = U.shape[1]
+    Ur = U.reshape((kx, ky, kz, NRx, nc), order="F")
+    pz0, pz1 = int(math.ceil((imsize[2] - kz) / 2)), int(math.floor((imsize[2] - kz) / 2))
+    Uz = np.pad(Ur, [(0, 0), (0, 0), (pz0, pz1), (0, 0), (0, 0)])
+    Uz = np.fft.fftshift(np.fft.ifft(np.fft.ifftshift(Uz, axes=2), axis=2), axes=2)
+    px0, px1 = int(math.ceil((imsize[0] - kx) / 2)), int(math.floor((imsize[0] - kx) / 2))
+    py0, py1 = int(math.ceil((imsize[1] - ky) / 2)), int(math.floor((imsize[1] - ky) / 2))
+    norm_factor = np.prod(imsize) / np.sqrt(np.prod(kernel_size))
+    for z in range(imsize[2]):
+        t = np.pad(Uz[:, :, z], [(px0, px1), (py0, py1), (0, 0), (0, 0)])
+        t = np.fft.fftshift(np.fft.ifftn(np.fft.ifftshift(t, axes=(0, 1)), axes=(0, 1)), axes=(0, 1))
+        M = t.reshape(-1, NRx, nc)
+        C = M @ np.conj(np.transpose(M, (0, 2, 1)))
+        w, v = np.linalg.eigh(C)
+        imax = np.argmax(w.real, axis=1)
+        pv = np.take_along_axis(v, imax[:, None, None], axis=2)[:, :, 0]
+        mask = np.sqrt(np.maximum(w[:, -1], 0)) * norm_factor > mask_thresh
+        sl = np.zeros((imsize[0] * imsize[1], NRx), dtype=np.complex128)
+        sl[mask] = pv[mask]
+        sens[:, :, z] = sl.reshape(imsize[0], imsize[1], NRx)
+    return sens * np.exp(-1j * np.angle(sens[..., 0:1]))
 
 
 # ---------------------------------------------------------------------------

@@ -567,3 +567,58 @@ def test_coil_compression_index_rule_and_subspace():
     v = A[0].conj()
     assert np.allclose(P @ v, S[0] * v, rtol=1e-9, atol=1e-9 * S[0])
     assert np.allclose(cc[0], np.einsum("c,cxyz->xyz", A[0], mps))
+
+
+def test_hankel_structure_single_sample():
+    calib = np.zeros((7, 6, 5, 3), dtype=np.complex128)
+    calib[4, 2, 3, 1] = 2 - 1j
+    H = O.fold_rx(O.hankel_fwd(calib, (3, 3, 3)))
+    K = 27
+    nbx, nby = 5, 4
+    rows, cols = np.nonzero(H)
+    assert len(rows) > 0
+    for r, b in zip(rows, cols):
+        k, c = r % K, r // K
+        ix, iy, iz = k % 3, (k // 3) % 3, k // 9  # Fortran order inside the block
+        bx, by, bz = b % nbx, (b // nbx) % nby, b // (nbx * nby)  # kx_start fastest
+        assert c == 1 and (bx + ix, by + iy, bz + iz) == (4, 2, 3)
+        assert H[r, b] == 2 - 1j
+    # every block that covers the sample has exactly one entry
+    covering = sum(1 for bz in range(3) for by in range(4) for bx in range(5)
+                   if 0 <= 4 - bx < 3 and 0 <= 2 - by < 3 and 0 <= 3 - bz < 3)
+    assert len(rows) == covering
+
+
+def _espirit_data(imsize=(24, 20, 16), nc=4, cal=(12, 12, 10), seed=80):
+    """k-space of a smooth object seen through smooth known maps; the
+    centred calibration region of size ``cal``."""
+    rng = np.random.default_rng(seed)
+    g = np.meshgrid(*[np.linspace(-1, 1, n) for n in imsize], indexing="ij")
+    obj = np.exp(-(g[0] ** 2 + g[1] ** 2 + g[2] ** 2) / 0.5) * (1 + 0.3 * g[0])
+    maps = []
+    for c in range(nc):
+        a = 2 * np.pi * c / nc
+        maps.append(np.exp(-((g[0] - 0.8 * np.cos(a)) ** 2 + (g[1] - 0.8 * np.sin(a)) ** 2) / 1.5)
+                    * np.exp(1j * (0.4 * g[2] + 0.3 * c)))
+    maps = np.stack(maps)
+    maps /= np.sqrt(np.sum(np.abs(maps) ** 2, axis=0))
+    ksp = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(maps * obj, axes=(1, 2, 3)), axes=(1, 2, 3)), axes=(1, 2, 3))
+    ksp += 1e-6 * np.abs(ksp).max() * (rng.standard_normal(ksp.shape) + 1j * rng.standard_normal(ksp.shape))
+    sl = tuple(slice((n - m) // 2, (n - m) // 2 + m) for n, m in zip(imsize, cal))
+    calib = np.transpose(ksp[(slice(None),) + sl], (1, 2, 3, 0))
+    return calib, maps, obj
+
+
+def test_espirit_recovers_known_maps():
+    imsize = (24, 20, 16)
+    calib, maps, obj = _espirit_data(imsize)
+    info = {}
+    sens = O.rx_espirit_3d(calib, imsize, kernel_size=(3, 3, 3), info=info)
+    assert 0 < info["kept"] < calib.shape[3] * 27
+    inner = obj > 0.3 * obj.max()
+    s = sens[inner]  # (n, nc)
+    t = np.transpose(maps, (1, 2, 3, 0))[inner]
+    corr = np.abs(np.sum(np.conj(s) * t, axis=1)) / (np.linalg.norm(s, axis=1) * np.linalg.norm(t, axis=1))
+    assert np.median(corr) > 0.999 and corr.min() > 0.98
+    assert np.allclose(np.linalg.norm(s, axis=1), 1, atol=1e-9)
+    assert np.all(np.abs(s[:, 0].imag) < 1e-9) and np.all(s[:, 0].real >= 0)
