@@ -298,6 +298,56 @@ int me_hermitian_eig(const double* A, int n, int max_sweeps, double* eigvals, do
  * (noise_whiten.py:11-12) and the compression matrix (coil_compression.py:38-43). */
 int me_coil_mix(const void* M, int cout, int cin, const void* x, long long n, void* y, hipStream_t stream);
 
+/* ---------------------------------------------------------------- ESPIRiT
+ * rx_espirit_3d (espirit.py:47-239) in four steps: calibration matrix,
+ * its Gram matrix, Jacobi eigendecomposition of the Gram matrix (the SVD of
+ * espirit.py:114), then the maps from the projector onto the kept subspace.
+ * calib_dims = {NKx, NKy, NKz, NRx}, kernel = {kx, ky, kz}, imsize = {Nx, Ny, Nz}. */
+
+/* Complex64 elements of the calibration matrix (kx ky kz NRx rows, one column
+ * per kernel-sized block); 0 when the kernel does not fit. */
+long long me_espirit_hankel_size(const int* calib_dims, const int* kernel);
+
+/* H (row-major) from calib (NKx, NKy, NKz, NRx) complex64: row k + K c,
+ * k = ix + kx (iy + ky iz), columns ordered x block start fastest
+ * (espirit.py:9-45: hankel_fwd then fold_rx). */
+int me_espirit_calibration_matrix(const void* calib, const int* calib_dims, const int* kernel, void* H,
+                                  hipStream_t stream);
+
+/* G[i][j] = sum_b X[i][b] conj(X[j][b]) for X m x n complex64 row-major,
+ * G m x m interleaved FP64. */
+int me_gram_rows(const void* X, int m, long long n, double* G, hipStream_t stream);
+
+/* Doubles of the workspace of me_jacobi_eig. */
+long long me_jacobi_workspace_size(int m);
+
+/* In-place parallel two-sided Jacobi on the Hermitian G (m x m interleaved
+ * FP64, overwritten by its diagonalised form): eigvals[i] = eigenvalue i
+ * (unsorted), column i of V (m x m) its eigenvector.  Sweeps until the
+ * off-diagonal Frobenius norm is <= tol times the diagonal one or
+ * max_sweeps; *sweeps_done (nullable) receives the count.  Synchronises
+ * the stream once per sweep. */
+int me_jacobi_eig(double* G, double* V, int m, int max_sweeps, double tol, double* eigvals, double* work,
+                  int* sweeps_done, hipStream_t stream);
+
+/* P = V_r V_r^H for the r columns idx[0..r) (device ints) of V (m x m
+ * interleaved FP64): the projector onto the kept singular subspace of
+ * espirit.py:118-137.  Xwork holds m x r complex64. */
+int me_espirit_projector(const double* V, int m, const int* idx, int r, void* Xwork, double* P,
+                         hipStream_t stream);
+
+/* Complex64 elements of the workspace of me_espirit_maps. */
+long long me_espirit_maps_workspace_size(int NRx, const int* kernel, const int* imsize);
+
+/* Maps (Nx, Ny, Nz, NRx) complex64 from the projector P (m = kx ky kz NRx):
+ * per voxel the principal eigenvector of the coil matrix of the zero-padded
+ * centred inverse FFT of the kept kernels (espirit.py:146-198), zero where
+ * sqrt(lambda_max) * prod(imsize) / sqrt(prod(kernel)) <= mask_thresh, with
+ * the phase of coil 0 removed (espirit.py:200-239).  Power iteration of at
+ * most iters steps.  NRx <= 64. */
+int me_espirit_maps(const double* P, int NRx, const int* kernel, const int* imsize, double mask_thresh, int iters,
+                    void* work, void* sens, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

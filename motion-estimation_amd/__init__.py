@@ -25,3 +25,4 @@ from .pyramid import pyramid_reconstruction, aggregate_sampling_masks, aggregate
 from .autograd import DataConsistencyLoss, data_consistency_loss  # noqa: F401
 from .sense import SenseRecon, sense_recon  # noqa: F401
 from .calibration import coil_compression, noise_whiten  # noqa: F401
+from .espirit import rx_espirit_3d  # noqa: F401

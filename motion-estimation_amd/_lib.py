@@ -51,6 +51,11 @@ _SIGS = {
     "me_whitening_matrix": [vp, i32, C.c_double, vp, vp],
     "me_hermitian_eig": [vp, i32, i32, vp, vp, vp],
     "me_coil_mix": [vp, i32, i32, vp, i64, vp, vp],
+    "me_espirit_calibration_matrix": [vp, vp, vp, vp, vp],
+    "me_gram_rows": [vp, i32, i64, vp, vp],
+    "me_jacobi_eig": [vp, vp, i32, i32, C.c_double, vp, vp, vp, vp],
+    "me_espirit_projector": [vp, i32, vp, i32, vp, vp, vp],
+    "me_espirit_maps": [vp, i32, vp, vp, C.c_double, i32, vp, vp, vp],
     "me_rigid_deriv_apply": [vp, i64, vp, i64, vp, vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp,
                              i32, i32, vp],
     "me_mask_mul": [vp, vp, i64, vp],
@@ -68,6 +73,9 @@ _SIZES = {
     "me_encode_coil_groups": ([i32], C.c_int),
     "me_dot_batched_partials_size": ([i32], C.c_longlong),
     "me_coil_gram_partials_size": ([i32], C.c_longlong),
+    "me_espirit_hankel_size": ([vp, vp], C.c_longlong),
+    "me_jacobi_workspace_size": ([i32], C.c_longlong),
+    "me_espirit_maps_workspace_size": ([i32, vp, vp], C.c_longlong),
 }
 _OPTIONAL = {}
 

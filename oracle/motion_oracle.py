@@ -690,6 +690,7 @@ def rx_espirit_3d(calib, imsize, kernel_size=(5, 5, 5), eig_thresh=0.02, mask_th
     px0, px1 = int(math.ceil((imsize[0] - kx) / 2)), int(math.floor((imsize[0] - kx) / 2))
     py0, py1 = int(math.ceil((imsize[1] - ky) / 2)), int(math.floor((imsize[1] - ky) / 2))
     norm_factor = np.prod(imsize) / np.sqrt(np.prod(kernel_size))
+    lam_max = np.zeros(imsize)
     for z in range(imsize[2]):
         t = np.pad(Uz[:, :, z], [(px0, px1), (py0, py1), (0, 0), (0, 0)])
         t = np.fft.fftshift(np.fft.ifftn(np.fft.ifftshift(t, axes=(0, 1)), axes=(0, 1)), axes=(0, 1))
@@ -699,9 +700,12 @@ def rx_espirit_3d(calib, imsize, kernel_size=(5, 5, 5), eig_thresh=0.02, mask_th
         imax = np.argmax(w.real, axis=1)
         pv = np.take_along_axis(v, imax[:, None, None], axis=2)[:, :, 0]
         mask = np.sqrt(np.maximum(w[:, -1], 0)) * norm_factor > mask_thresh
+        lam_max[:, :, z] = (np.sqrt(np.maximum(w[:, -1], 0)) * norm_factor).reshape(imsize[0], imsize[1])
         sl = np.zeros((imsize[0] * imsize[1], NRx), dtype=np.complex128)
         sl[mask] = pv[mask]
         sens[:, :, z] = sl.reshape(imsize[0], imsize[1], NRx)
+    if info is not None:
+        info["mask_value"] = lam_max  # sqrt(lambda_max) * norm_factor per voxel
     return sens * np.exp(-1j * np.angle(sens[..., 0:1]))
 
 

@@ -1,0 +1,143 @@
+"""GPU parity of ESPIRiT (espirit.py:9-239) against the FP64 oracle, and of
+its building blocks (calibration matrix, FP64 Gram, parallel Jacobi
+eigensolver) against NumPy.
+
+The maps are compared voxel by voxel after the reference's own phase
+normalisation (coil 0 real and non-negative), which makes the principal
+eigenvector unique; voxels whose mask value sqrt(lambda_max) * prod(N) /
+sqrt(prod(kernel)) lies within 1e-4 (relative) of the threshold are left
+out of the mask comparison, where FP32 and FP64 may round either way."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from espirit_data import espirit_data
+from oracle import motion_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda:0")
+
+
+def _ints(v):
+    return (C.c_int * len(v))(*[int(x) for x in v])
+
+
+def test_calibration_matrix_matches_oracle(me):
+    rng = np.random.default_rng(3)
+    calib = (rng.standard_normal((7, 6, 5, 3)) + 1j * rng.standard_normal((7, 6, 5, 3))).astype(np.complex64)
+    kernel = (3, 2, 4)
+    ref = O.fold_rx(O.hankel_fwd(calib.astype(np.complex128), kernel))
+    dims, kern = _ints(calib.shape), _ints(kernel)
+    n = me._lib.size("me_espirit_hankel_size", dims, kern)
+    assert n == ref.size
+    H = torch.empty(n, dtype=torch.complex64, device=DEV)
+    me._lib.call("me_espirit_calibration_matrix", me._lib.ptr(torch.as_tensor(calib, device=DEV)), dims, kern,
+                 me._lib.ptr(H), me._lib.stream(DEV))
+    assert np.array_equal(H.cpu().numpy().reshape(ref.shape), ref.astype(np.complex64))
+    assert me._lib.size("me_espirit_hankel_size", dims, _ints((8, 2, 2))) == 0
+
+
+@pytest.mark.parametrize("m,n", [(1, 5), (33, 70), (100, 40)])
+def test_gram_rows_matches_numpy(me, m, n):
+    rng = np.random.default_rng(m)
+    X = (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))).astype(np.complex64)
+    G = torch.empty((m, m, 2), dtype=torch.float64, device=DEV)
+    me._lib.call("me_gram_rows", me._lib.ptr(torch.as_tensor(X, device=DEV)), m, n, me._lib.ptr(G),
+                 me._lib.stream(DEV))
+    Xd = X.astype(np.complex128)
+    ref = Xd @ Xd.conj().T
+    got = torch.view_as_complex(G).cpu().numpy()
+    assert np.linalg.norm(got - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+def _jacobi(me, A, sweeps=40, tol=1e-14):
+    m = A.shape[0]
+    G = torch.view_as_real(torch.as_tensor(A, device=DEV)).contiguous()
+    V = torch.empty((m, m, 2), dtype=torch.float64, device=DEV)
+    lam = torch.empty(m, dtype=torch.float64, device=DEV)
+    work = torch.empty(me._lib.size("me_jacobi_workspace_size", m), dtype=torch.float64, device=DEV)
+    done = C.c_int(0)
+    me._lib.call("me_jacobi_eig", me._lib.ptr(G), me._lib.ptr(V), m, sweeps, tol, me._lib.ptr(lam),
+                 me._lib.ptr(work), C.byref(done), me._lib.stream(DEV))
+    return lam.cpu().numpy(), torch.view_as_complex(V).cpu().numpy(), done.value
+
+
+@pytest.mark.parametrize("m", [1, 2, 5, 64, 127])
+def test_jacobi_eig_matches_eigh(me, m):
+    rng = np.random.default_rng(10 + m)
+    X = rng.standard_normal((m, m + 3)) + 1j * rng.standard_normal((m, m + 3))
+    A = X @ X.conj().T
+    lam, V, sweeps = _jacobi(me, A)
+    ref = np.linalg.eigvalsh(A)
+    assert np.max(np.abs(np.sort(lam) - ref)) <= 1e-12 * ref[-1]
+    assert np.linalg.norm(V @ np.diag(lam) @ V.conj().T - A) <= 1e-12 * np.linalg.norm(A)
+    assert np.linalg.norm(V.conj().T @ V - np.eye(m)) <= 1e-12 * m
+    assert sweeps < 40
+
+
+def test_jacobi_eig_rank_deficient(me):
+    """Rank 6 of 40 (the shape of a calibration Gram matrix with more rows
+    than blocks): the zero eigenvalues come out at the round-off level and
+    the projector onto the leading 6 equals X's column-space projector."""
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((40, 6)) + 1j * rng.standard_normal((40, 6))
+    A = X @ X.conj().T
+    lam, V, _ = _jacobi(me, A)
+    order = np.argsort(-lam)
+    assert np.all(np.abs(lam[order[6:]]) <= 1e-12 * lam[order[0]])
+    U = V[:, order[:6]]
+    Q, _ = np.linalg.qr(X)
+    assert np.linalg.norm(U @ U.conj().T - Q @ Q.conj().T) <= 1e-10
+
+
+def _compare_maps(sens, ref, info, mask_thresh):
+    got = sens.cpu().numpy().astype(np.complex128)
+    val = info["mask_value"]
+    border = np.abs(val - mask_thresh) <= 1e-4 * max(mask_thresh, 1e-30)
+    m_ref = np.any(ref != 0, axis=-1)
+    m_got = np.any(got != 0, axis=-1)
+    assert np.array_equal(m_ref[~border], m_got[~border])
+    both = m_ref & m_got
+    assert both.sum() > 0
+    err = np.linalg.norm(got[both] - ref[both]) / np.linalg.norm(ref[both])
+    assert err <= 1e-4, err
+    # unit vectors, coil 0 real and non-negative
+    assert np.allclose(np.linalg.norm(got[both], axis=-1), 1.0, atol=1e-5)
+    assert np.all(np.abs(got[both][:, 0].imag) <= 1e-6) and np.all(got[both][:, 0].real >= 0)
+
+
+@pytest.mark.parametrize("imsize,nc,cal,kernel,eig_thresh,mask_thresh", [
+    ((24, 20, 16), 4, (12, 12, 10), (3, 3, 3), 0.02, 0.99),
+    ((25, 19, 15), 4, (11, 12, 9), (3, 2, 3), 0.02, 0.9),
+    ((32, 28, 20), 8, (14, 14, 12), (5, 5, 5), 0.02, 0.99),
+    ((24, 20, 16), 6, (12, 12, 10), (3, 3, 3), 20, 0.5),
+])
+def test_espirit_matches_oracle(me, imsize, nc, cal, kernel, eig_thresh, mask_thresh):
+    calib, _, _ = espirit_data(imsize, nc=nc, cal=cal, seed=sum(imsize) + nc)
+    calib = calib.astype(np.complex64)
+    info = {}
+    ref = O.rx_espirit_3d(calib, imsize, kernel_size=kernel, eig_thresh=eig_thresh, mask_thresh=mask_thresh,
+                          info=info)
+    sens = me.rx_espirit_3d(calib, imsize, kernel_size=kernel, eig_thresh=eig_thresh, mask_thresh=mask_thresh)
+    assert me.rx_espirit_3d.kept == info["kept"]
+    S = info["singular_values"]
+    sv = me.rx_espirit_3d.singular_values
+    k = min(S.size, sv.size)
+    assert np.max(np.abs(sv[:k] - S[:k])) <= 1e-6 * S[0]
+    _compare_maps(sens, ref, info, mask_thresh)
+
+
+def test_espirit_nothing_kept_gives_zero_maps(me):
+    calib = np.zeros((8, 8, 8, 3), dtype=np.complex64)
+    sens = me.rx_espirit_3d(calib, (10, 10, 10), kernel_size=(3, 3, 3))
+    assert sens.shape == (10, 10, 10, 3) and not torch.any(sens != 0)
+    assert me.rx_espirit_3d.kept == 0
+
+
+def test_espirit_rejects_oversized_kernel(me):
+    calib = np.ones((4, 4, 4, 2), dtype=np.complex64)
+    with pytest.raises(ValueError):
+        me.rx_espirit_3d(calib, (8, 8, 8), kernel_size=(5, 5, 5))
