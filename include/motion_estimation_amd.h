@@ -326,7 +326,7 @@ long long me_jacobi_workspace_size(int m);
  * (unsorted), column i of V (m x m) its eigenvector.  Sweeps until the
  * off-diagonal Frobenius norm is <= tol times the diagonal one or
  * max_sweeps; *sweeps_done (nullable) receives the count.  Synchronises
- * the stream once per sweep. */
+ * the stream once per sweep; work: me_jacobi_workspace_size(m) doubles. */
 int me_jacobi_eig(double* G, double* V, int m, int max_sweeps, double tol, double* eigvals, double* work,
                   int* sweeps_done, hipStream_t stream);
 

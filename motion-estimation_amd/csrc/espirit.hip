@@ -184,9 +184,9 @@ __device__ __forceinline__ void rot_rows(const Rot& J, double& pr, double& pi, d
 }
 
 // One Jacobi round.  blockIdx.y < npairs: rows (p_a, q_a) of G against the
-// column pair b of this thread; blockIdx.y >= npairs: row blockIdx.y - npairs
-// of V.  Consecutive b give column indices in two contiguous runs, so the
-// row accesses coalesce.
+// column pair b of this thread; blockIdx.y >= npairs (launched only when V
+// rows do not fit LDS): row blockIdx.y - npairs of V.  Consecutive b give
+// column indices in two contiguous runs, so the row accesses coalesce.
 __global__ void __launch_bounds__(256) k_jac_round(double* __restrict__ G, double* __restrict__ V, int m, int mm,
                                                    int r, const double* __restrict__ rot) {
   const int npairs = mm / 2;
@@ -241,6 +241,36 @@ __global__ void __launch_bounds__(256) k_jac_round(double* __restrict__ G, doubl
     vq[0] = qr;
     vq[1] = qi;
   }
+}
+
+// V <- V J_r0 ... J_{r0+nr-1}: one workgroup per row of V, the row held in
+// LDS across nr rounds (rotation parameters of round r0 + k at
+// rots + k * npairs * 4), so V makes one HBM round trip per nr rounds.
+constexpr int kRoundsPerVPass = 32;
+constexpr int kMaxLdsRow = 8192;  // complex FP64 entries of a row in LDS
+__global__ void __launch_bounds__(256) k_jac_apply_v(double* __restrict__ V, int m, int mm, int r0, int nr,
+                                                     const double* __restrict__ rots) {
+  extern __shared__ double2 vrow[];
+  double2* gv = reinterpret_cast<double2*>(V) + (long long)blockIdx.x * m;
+  for (int j = threadIdx.x; j < m; j += blockDim.x) vrow[j] = gv[j];
+  __syncthreads();
+  const int npairs = mm / 2;
+  for (int k = 0; k < nr; ++k) {
+    const double* rot = rots + (long long)k * npairs * 4;
+    for (int b = threadIdx.x; b < npairs; b += blockDim.x) {
+      int p, q;
+      rr_pair(mm, r0 + k, b, p, q);
+      if (q >= m) continue;
+      const Rot J = load_rot(rot, b);
+      if (is_identity(J)) continue;
+      double2 xp = vrow[p], xq = vrow[q];
+      rot_cols(J, xp.x, xp.y, xq.x, xq.y);
+      vrow[p] = xp;
+      vrow[q] = xq;
+    }
+    __syncthreads();
+  }
+  for (int j = threadIdx.x; j < m; j += blockDim.x) gv[j] = vrow[j];
 }
 
 constexpr int kNormBlocks = 256;
@@ -437,7 +467,11 @@ __global__ void __launch_bounds__(kVoxThreads) k_espirit_voxels(const float2* __
       double inv = nrm > 0.0 ? 1.0 / sqrt(nrm) : 0.0;
       vx *= inv;
       vy *= inv;
-      double prev = 1.0;
+      // Each iterate is rescaled to unit norm with its coil-0 entry real and
+      // non-negative: the FP32 coil matrix is Hermitian only to rounding, so
+      // its dominant eigenvalue may carry a tiny phase that would otherwise
+      // rotate the iterate every step and defeat the stopping test.
+      double prev = 1.0, lam = 0.0;
       for (int it = 0; it < iters; ++it) {
         vg[lane] = make_double2(vx, vy);
         __builtin_amdgcn_wave_barrier();
@@ -452,41 +486,34 @@ __global__ void __launch_bounds__(kVoxThreads) k_espirit_voxels(const float2* __
           }
         __builtin_amdgcn_wave_barrier();
         double wn = wx * wx + wy * wy;
+        double rq = vx * wx + vy * wy;  // Re(v^H C v), v of unit norm
 #pragma unroll
-        for (int o = G2 / 2; o > 0; o >>= 1) wn += __shfl_xor(wn, o, G2);
+        for (int o = G2 / 2; o > 0; o >>= 1) {
+          wn += __shfl_xor(wn, o, G2);
+          rq += __shfl_xor(rq, o, G2);
+        }
+        lam = rq;
+        const double w0x = __shfl(wx, 0, G2), w0y = __shfl(wy, 0, G2);
+        const double a0 = sqrt(w0x * w0x + w0y * w0y);
+        double px = 1.0, py = 0.0;
+        if (a0 > 0.0) {
+          px = w0x / a0;
+          py = -w0y / a0;
+        }
         inv = wn > 0.0 ? 1.0 / sqrt(wn) : 0.0;
-        wx *= inv;
-        wy *= inv;
-        double dif = (wx - vx) * (wx - vx) + (wy - vy) * (wy - vy);
+        const double nx = (wx * px - wy * py) * inv, ny = (wx * py + wy * px) * inv;
+        double dif = (nx - vx) * (nx - vx) + (ny - vy) * (ny - vy);
 #pragma unroll
         for (int o = G2 / 2; o > 0; o >>= 1) dif += __shfl_xor(dif, o, G2);
-        vx = wx;
-        vy = wy;
+        vx = nx;
+        vy = ny;
         dif = sqrt(dif);
         const double rho = fmin(dif / fmax(prev, 1e-300), 0.999999);
         prev = dif;
         if (dif * rho <= 1e-10 * (1.0 - rho) || dif <= 1e-14) break;  // group-uniform
       }
-      // Rayleigh quotient of the final unit vector: lambda_max(Cs)
-      vg[lane] = make_double2(vx, vy);
-      __builtin_amdgcn_wave_barrier();
-      double wx = 0.0, wy = 0.0;
-#pragma unroll
-      for (int c2 = 0; c2 < G2; ++c2)
-        if (c2 < NRx) {
-          const double2 u = vg[c2];
-          const double ax = row[c2].x, ay = row[c2].y;
-          wx = fma(ax, u.x, fma(-ay, u.y, wx));
-          wy = fma(ax, u.y, fma(ay, u.x, wy));
-        }
-      __builtin_amdgcn_wave_barrier();
-      double rq = vx * wx + vy * wy;
-#pragma unroll
-      for (int o = G2 / 2; o > 0; o >>= 1) rq += __shfl_xor(rq, o, G2);
-      const double v0x = __shfl(vx, 0, G2), v0y = __shfl(vy, 0, G2);
-      const double a0 = sqrt(v0x * v0x + v0y * v0y);
-      const double px = a0 > 0.0 ? v0x / a0 : 1.0, py = a0 > 0.0 ? -v0y / a0 : 0.0;
-      if (rq > (double)thr2k) outv = make_float2((float)(vx * px - vy * py), (float)(vx * py + vy * px));
+      // lam: Rayleigh quotient of the last iterate, lambda_max(Cs)
+      if (lam > (double)thr2k) outv = make_float2((float)vx, (float)vy);
     }
     if (act) sens[(((long long)x * Ny + y) * Nz + z) * NRx + lane] = outv;
   }
@@ -546,7 +573,9 @@ int me_gram_rows(const void* X, int m, long long n, double* G, hipStream_t strea
   return 0;
 }
 
-long long me_jacobi_workspace_size(int m) { return (long long)(m + 2) / 2 * 4 + 2 * kNormBlocks; }
+long long me_jacobi_workspace_size(int m) {
+  return (long long)kRoundsPerVPass * ((m + 2) / 2) * 4 + 2 * kNormBlocks;
+}
 
 int me_jacobi_eig(double* G, double* V, int m, int max_sweeps, double tol, double* eigvals, double* work,
                   int* sweeps_done, hipStream_t stream) {
@@ -554,8 +583,13 @@ int me_jacobi_eig(double* G, double* V, int m, int max_sweeps, double tol, doubl
   ME_REQUIRE(m <= 32768, "jacobi_eig: at most 32768 rows");
   const int mm = m + (m & 1);
   const int npairs = mm / 2;
-  double* rot = work;
-  double* part = work + (long long)npairs * 4;
+  const bool lds_v = m <= kMaxLdsRow;
+  double* rots = work;  // kRoundsPerVPass rounds of npairs x 4
+  double* part = work + (long long)kRoundsPerVPass * npairs * 4;
+  const size_t vshm = (size_t)m * sizeof(double2);
+  if (lds_v)
+    ME_TRY_HIP(hipFuncSetAttribute((const void*)k_jac_apply_v, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)vshm));
   hipLaunchKernelGGL(k_identity, dim3(grid_for((long long)m * m)), dim3(256), 0, stream, V, m);
   ME_TRY_HIP(hipGetLastError());
   int sw = 0;
@@ -571,11 +605,18 @@ int me_jacobi_eig(double* G, double* V, int m, int max_sweeps, double tol, doubl
       dia += hp[2 * b + 1];
     }
     if (off <= tol * tol * dia) break;
-    const dim3 grid((unsigned)((npairs + 255) / 256), (unsigned)(npairs + m));
-    for (int r = 0; r < mm - 1; ++r) {
-      hipLaunchKernelGGL(k_jac_params, dim3((npairs + 255) / 256), dim3(256), 0, stream, (const double*)G, m, mm, r,
-                         rot);
-      hipLaunchKernelGGL(k_jac_round, grid, dim3(256), 0, stream, G, V, m, mm, r, (const double*)rot);
+    const dim3 grid((unsigned)((npairs + 255) / 256), (unsigned)(npairs + (lds_v ? 0 : m)));
+    for (int r0 = 0; r0 < mm - 1; r0 += kRoundsPerVPass) {
+      const int nr = (mm - 1 - r0) < kRoundsPerVPass ? (mm - 1 - r0) : kRoundsPerVPass;
+      for (int k = 0; k < nr; ++k) {
+        double* rot = rots + (long long)k * npairs * 4;
+        hipLaunchKernelGGL(k_jac_params, dim3((npairs + 255) / 256), dim3(256), 0, stream, (const double*)G, m, mm,
+                           r0 + k, rot);
+        hipLaunchKernelGGL(k_jac_round, grid, dim3(256), 0, stream, G, V, m, mm, r0 + k, (const double*)rot);
+      }
+      if (lds_v)
+        hipLaunchKernelGGL(k_jac_apply_v, dim3((unsigned)m), dim3(256), vshm, stream, V, m, mm, r0, nr,
+                           (const double*)rots);
     }
     ME_TRY_HIP(hipGetLastError());
   }

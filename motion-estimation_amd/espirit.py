@@ -19,7 +19,7 @@ import torch
 from . import _lib
 from .utils import as_c64
 
-JACOBI_TOL = 1e-14
+JACOBI_TOL = 1e-12
 
 
 def _ints(vals):
