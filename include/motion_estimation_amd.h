@@ -103,6 +103,48 @@ int me_fft_axis(const void* x, long long x_bstride, void* y, long long y_bstride
                 const long long* st, int axis, int inverse, hipStream_t stream);
 
 
+/* ---------------------------------------------------------------- tuning
+ * Lines per workgroup of the plain line pass (one input, one output, FFT ->
+ * phase -> inverse FFT; the bulk of the rigid-transform passes): 8, 16
+ * (default) or 32, kept in a process-wide table keyed by the line geometry
+ * (line length, strides, tile extents) of a volume with extents n and element
+ * strides st along `axis`.  No counterpart in the reference (CuPy picks its
+ * own FFT plans); results are bitwise independent of the choice. */
+
+/* Set the lines per workgroup for passes along `axis` of this geometry
+ * (0 restores the default).  Errors if that count is not built for the line
+ * length (8 and 32 exist up to 512 threads per workgroup). */
+int me_line_tuning_set(const int* n, const long long* st, int axis, int lines);
+
+/* Lines per workgroup in use for passes along `axis` of this geometry. */
+int me_line_tuning_get(const int* n, const long long* st, int axis);
+
+/* Forget every tuned entry. */
+void me_line_tuning_clear(void);
+
+/* Autotuner: for each axis, time `reps` plain passes x -> y (nbatch volumes
+ * at bstride elements; y may equal x, its contents are overwritten) with 8,
+ * 16 and 32 lines per workgroup using HIP events on `stream`, and keep the
+ * fastest in the table.  chosen[3] (nullable) receives the picks,
+ * times_ms[9] (nullable) the mean time per pass of each candidate (-1 where
+ * not built).  Synchronises the stream; not for use under graph capture. */
+int me_tune_line_passes(const void* x, void* y, long long bstride, int nbatch, const int* n, const long long* st,
+                        int reps, int* chosen, float* times_ms, hipStream_t stream);
+
+/* ---------------------------------------------------------------- graphs
+ * Thin wrappers over HIP stream capture for replaying a recorded sequence of
+ * the calls above (no counterpart in the reference, which launches eagerly
+ * from the host).  Between me_graph_begin and me_graph_end every call on the
+ * stream (and on streams joined to it through events) is recorded, not run;
+ * allocations on the capturing thread are errors. */
+int me_graph_begin(hipStream_t stream);
+/* Ends the capture and instantiates it: *exec receives the graph handle. */
+int me_graph_end(hipStream_t stream, void** exec);
+/* Ends a capture left open by an error, discarding it. */
+int me_graph_abort(hipStream_t stream);
+int me_graph_launch(void* exec, hipStream_t stream);
+int me_graph_destroy(void* exec);
+
 /* ---------------------------------------------------------------- encoding
  * Sample plan of a set of shots (host-built from the shot masks
  * (S, N0, N1, 1), simulation.py:396-402): samples (k0, k1) of every shot in one

@@ -26,3 +26,4 @@ from .autograd import DataConsistencyLoss, data_consistency_loss  # noqa: F401
 from .sense import SenseRecon, sense_recon  # noqa: F401
 from .calibration import coil_compression, noise_whiten  # noqa: F401
 from .espirit import rx_espirit_3d  # noqa: F401
+from .batched import BatchedJointRecon  # noqa: F401

@@ -52,6 +52,13 @@ _SIGS = {
     "me_hermitian_eig": [vp, i32, i32, vp, vp, vp],
     "me_coil_mix": [vp, i32, i32, vp, i64, vp, vp],
     "me_espirit_calibration_matrix": [vp, vp, vp, vp, vp],
+    "me_line_tuning_set": [vp, vp, i32, i32],
+    "me_tune_line_passes": [vp, vp, i64, i32, vp, vp, i32, vp, vp, vp],
+    "me_graph_begin": [vp],
+    "me_graph_end": [vp, vp],
+    "me_graph_abort": [vp],
+    "me_graph_launch": [vp, vp],
+    "me_graph_destroy": [vp],
     "me_gram_rows": [vp, i32, i64, vp, vp],
     "me_jacobi_eig": [vp, vp, i32, i32, C.c_double, vp, vp, vp, vp],
     "me_espirit_projector": [vp, i32, vp, i32, vp, vp, vp],
@@ -74,15 +81,18 @@ _SIZES = {
     "me_dot_batched_partials_size": ([i32], C.c_longlong),
     "me_coil_gram_partials_size": ([i32], C.c_longlong),
     "me_espirit_hankel_size": ([vp, vp], C.c_longlong),
+    "me_line_tuning_get": ([vp, vp, i32], C.c_int),
     "me_jacobi_workspace_size": ([i32], C.c_longlong),
     "me_espirit_maps_workspace_size": ([i32, vp, vp], C.c_longlong),
 }
 _OPTIONAL = {}
+# entry points returning nothing
+_VOID = {"me_line_tuning_clear": []}
 
 
 def declared_symbols():
     """Names of every entry point the Python layer binds."""
-    return sorted(["me_last_error", *_SIGS.keys(), *_SIZES.keys(), *_OPTIONAL.keys()])
+    return sorted(["me_last_error", *_SIGS.keys(), *_SIZES.keys(), *_OPTIONAL.keys(), *_VOID.keys()])
 
 
 def load():
@@ -100,6 +110,10 @@ def load():
         for name, args in {**_SIGS, **_OPTIONAL}.items():
             fn = getattr(lib, name)
             fn.restype = C.c_int
+            fn.argtypes = args
+        for name, args in _VOID.items():
+            fn = getattr(lib, name)
+            fn.restype = None
             fn.argtypes = args
         for name, (args, res) in _SIZES.items():
             fn = getattr(lib, name)

@@ -19,24 +19,25 @@
 #include "me_device.h"
 #include "me_internal.h"
 
+#include <map>
+#include <mutex>
+#include <tuple>
+
 namespace me {
 
-// Lines per workgroup: 16 for both layouts (strided lanes line-fastest:
-// 128-byte contiguous segments per line position).  32 strided lines measured
-// 5% faster along axis 0 and 10% slower along axis 2 of a 256x256x192
-// plane-major volume.
-#ifndef ME_STRIDED_LINES
-#define ME_STRIDED_LINES 16
-#endif
-constexpr int kLinesContig = 16;
-constexpr int kLinesStrided = ME_STRIDED_LINES;
+// Lines per workgroup LL: 16 by default for both layouts (strided lanes
+// line-fastest: 128-byte contiguous segments per line position).  The plain
+// single-buffer pass (one input, one output, FFT-phase-inverse FFT) is also
+// built for 8 and 32 lines, picked per geometry by the tuning table below
+// (me_tune_line_passes times the variants on the caller's buffers).
+constexpr int kLinesDefault = 16;
 
-template <int N, bool CONTIG>
+template <int N, bool CONTIG, int LL = kLinesDefault>
 struct LineT {
   static constexpr int R1 = LineFactor<N>::R1;
   static constexpr int R2 = LineFactor<N>::R2;
   static constexpr int T = R1 > R2 ? R1 : R2;
-  static constexpr int L = (CONTIG || T > 16) ? kLinesContig : kLinesStrided;
+  static constexpr int L = LL;
   static constexpr int NT = T * L;
 };
 
@@ -94,9 +95,9 @@ __device__ __forceinline__ void phase_acc(const PhaseSpec& P, const double* cf, 
 // MODE 0: FFT, phase, inverse FFT; 1: forward FFT only (natural-order
 // spectrum out); 2: inverse FFT only (natural-order spectrum in).
 // REDUCE: sum the result over the batch and write batch entry 0 only.
-template <int N, bool CONTIG, int NIN, int NOUT, int MODE, bool REDUCE>
-__global__ void __launch_bounds__((LineT<N, CONTIG>::NT)) k_linepass(LineArgs A) {
-  using LT = LineT<N, CONTIG>;
+template <int N, bool CONTIG, int NIN, int NOUT, int MODE, bool REDUCE, int LL = kLinesDefault>
+__global__ void __launch_bounds__((LineT<N, CONTIG, LL>::NT)) k_linepass(LineArgs A) {
+  using LT = LineT<N, CONTIG, LL>;
   constexpr int R1 = LT::R1;
   constexpr int R2 = LT::R2;
   constexpr int T = LT::T;
@@ -265,53 +266,84 @@ __global__ void __launch_bounds__((LineT<N, CONTIG>::NT)) k_linepass(LineArgs A)
 }
 
 namespace {
+// Tuning table: lines per workgroup of the plain pass, keyed by the line
+// geometry of a launch.  Process-wide, filled by me_tune_line_passes or
+// me_line_tuning_set; absent keys use kLinesDefault.
+struct LineKey {
+  int n, contig, n_u, n_v;
+  long long st_a, st_u, st_v;
+  bool operator<(const LineKey& o) const {
+    return std::tie(n, contig, n_u, n_v, st_a, st_u, st_v) <
+           std::tie(o.n, o.contig, o.n_u, o.n_v, o.st_a, o.st_u, o.st_v);
+  }
+};
+std::mutex g_tune_mu;
+std::map<LineKey, int> g_tune;
+
+int tuned_lines(const LineKey& k) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(k);
+  return it == g_tune.end() ? kLinesDefault : it->second;
+}
+
+template <int N, bool CONTIG, int LL>
+constexpr bool lines_fit() {
+  using LT = LineT<N, CONTIG, LL>;
+  // at most 512 threads: 1024-thread variants of the long lines spill
+  return LT::NT <= 512 && (LT::L * LT::R1 * (LT::R2 + 1) + N) * 8 <= 160 * 1024;
+}
+
 template <bool CONTIG, int NIN, int NOUT, int MODE, bool REDUCE>
 struct LineLaunch {
   const LineArgs* args;
   dim3 grid;
   hipStream_t stream;
-  template <int N>
-  void operator()() {
-    const int L = LineT<N, CONTIG>::L;
-    const long long nblk = CONTIG ? ((long long)args->n_u * args->n_v + L - 1) / L
-                                  : (long long)((args->n_u + L - 1) / L) * args->n_v;
+  int lines;
+  template <int N, int LL>
+  void go() {
+    using LT = LineT<N, CONTIG, LL>;
+    const long long nblk = CONTIG ? ((long long)args->n_u * args->n_v + LT::L - 1) / LT::L
+                                  : (long long)((args->n_u + LT::L - 1) / LT::L) * args->n_v;
     dim3 g = grid;
     g.x = (unsigned)nblk;
-    hipLaunchKernelGGL((k_linepass<N, CONTIG, NIN, NOUT, MODE, REDUCE>), g, dim3(LineT<N, CONTIG>::NT), 0, stream,
-                       *args);
+    hipLaunchKernelGGL((k_linepass<N, CONTIG, NIN, NOUT, MODE, REDUCE, LL>), g, dim3(LT::NT), 0, stream, *args);
+  }
+  template <int N>
+  void operator()() {
+    if constexpr (NIN == 1 && NOUT == 1 && MODE == 0 && !REDUCE) {
+      if constexpr (lines_fit<N, CONTIG, 8>())
+        if (lines == 8) return go<N, 8>();
+      if constexpr (lines_fit<N, CONTIG, 32>())
+        if (lines == 32) return go<N, 32>();
+    }
+    go<N, kLinesDefault>();
   }
 };
 
 template <bool CONTIG, int NIN, int NOUT, int MODE, bool REDUCE>
-bool launch_n(int n, const LineArgs& a, dim3 grid, hipStream_t s) {
-  LineLaunch<CONTIG, NIN, NOUT, MODE, REDUCE> f{&a, grid, s};
+bool launch_n(int n, const LineArgs& a, dim3 grid, hipStream_t s, int lines) {
+  LineLaunch<CONTIG, NIN, NOUT, MODE, REDUCE> f{&a, grid, s, lines};
   return dispatch_line_length(n, f);
 }
 
 template <bool CONTIG>
-bool launch_io(int n, int nin, int nout, const LineArgs& a, dim3 grid, hipStream_t s) {
-  if (a.mode == 1) return nin == 1 && nout == 1 && launch_n<CONTIG, 1, 1, 1, false>(n, a, grid, s);
-  if (a.mode == 2) return nin == 1 && nout == 1 && launch_n<CONTIG, 1, 1, 2, false>(n, a, grid, s);
-  if (a.reduce) return nin == 1 && nout == 1 && launch_n<CONTIG, 1, 1, 0, true>(n, a, grid, s);
-  if (nin == 1 && nout == 1) return launch_n<CONTIG, 1, 1, 0, false>(n, a, grid, s);
-  if (nin == 1 && nout == 2) return launch_n<CONTIG, 1, 2, 0, false>(n, a, grid, s);
-  if (nin == 2 && nout == 2) return launch_n<CONTIG, 2, 2, 0, false>(n, a, grid, s);
-  if (nin == 2 && nout == 1) return launch_n<CONTIG, 2, 1, 0, false>(n, a, grid, s);
+bool launch_io(int n, int nin, int nout, const LineArgs& a, dim3 grid, hipStream_t s, int lines) {
+  if (a.mode == 1) return nin == 1 && nout == 1 && launch_n<CONTIG, 1, 1, 1, false>(n, a, grid, s, lines);
+  if (a.mode == 2) return nin == 1 && nout == 1 && launch_n<CONTIG, 1, 1, 2, false>(n, a, grid, s, lines);
+  if (a.reduce) return nin == 1 && nout == 1 && launch_n<CONTIG, 1, 1, 0, true>(n, a, grid, s, lines);
+  if (nin == 1 && nout == 1) return launch_n<CONTIG, 1, 1, 0, false>(n, a, grid, s, lines);
+  if (nin == 1 && nout == 2) return launch_n<CONTIG, 1, 2, 0, false>(n, a, grid, s, lines);
+  if (nin == 2 && nout == 2) return launch_n<CONTIG, 2, 2, 0, false>(n, a, grid, s, lines);
+  if (nin == 2 && nout == 1) return launch_n<CONTIG, 2, 1, 0, false>(n, a, grid, s, lines);
   return false;
 }
 }  // namespace
 
-int launch_linepass(const VolGeom& g, int axis, int nin, int nout, LineArgs A, const HostPhase ph[2][2],
-                    hipStream_t stream) {
-  ME_REQUIRE(axis >= 0 && axis < 3, "line pass: axis out of range");
-  ME_REQUIRE(nin >= 1 && nin <= 2 && nout >= 1 && nout <= 2, "line pass: 1 or 2 inputs/outputs");
-  ME_REQUIRE(!(A.reduce && (nout != 1 || nin != 1)), "line pass: batch reduction needs one input and output");
-  ME_REQUIRE(A.mode == 0 || (nin == 1 && nout == 1 && !A.reduce), "line pass: FFT-only modes are single-buffer");
-  ME_REQUIRE(A.nbatch >= 1, "line pass: empty batch");
+// Tile axes of a pass along ``axis``: au runs across the lines of a
+// workgroup, av across workgroups.  contig: the line axis has unit stride.
+static void line_layout(const VolGeom& g, int axis, bool& contig, int& au, int& av) {
   const int o1 = axis == 0 ? 1 : 0;
   const int o2 = axis == 2 ? 1 : 2;
-  int au, av;
-  bool contig;
   if (g.st[axis] == 1) {
     contig = true;
     au = o1;
@@ -333,6 +365,42 @@ int launch_linepass(const VolGeom& g, int axis, int nin, int nout, LineArgs A, c
       av = au == o1 ? o2 : o1;
     }
   }
+}
+
+static LineKey line_key(const VolGeom& g, int axis) {
+  bool contig;
+  int au, av;
+  line_layout(g, axis, contig, au, av);
+  return LineKey{g.n[axis], contig ? 1 : 0, g.n[au], g.n[av], g.st[axis], g.st[au], g.st[av]};
+}
+
+// Whether the plain pass of this line length and layout is built for `lines`.
+struct LinesBuilt {
+  bool contig;
+  int lines;
+  bool ok = false;
+  template <int N>
+  void operator()() {
+    if (lines == kLinesDefault) ok = true;
+    if (lines == 8) ok = contig ? lines_fit<N, true, 8>() : lines_fit<N, false, 8>();
+    if (lines == 32) ok = contig ? lines_fit<N, true, 32>() : lines_fit<N, false, 32>();
+  }
+};
+static bool lines_built(int n, bool contig, int lines) {
+  LinesBuilt f{contig, lines};
+  return dispatch_line_length(n, f) && f.ok;
+}
+
+int launch_linepass(const VolGeom& g, int axis, int nin, int nout, LineArgs A, const HostPhase ph[2][2],
+                    hipStream_t stream) {
+  ME_REQUIRE(axis >= 0 && axis < 3, "line pass: axis out of range");
+  ME_REQUIRE(nin >= 1 && nin <= 2 && nout >= 1 && nout <= 2, "line pass: 1 or 2 inputs/outputs");
+  ME_REQUIRE(!(A.reduce && (nout != 1 || nin != 1)), "line pass: batch reduction needs one input and output");
+  ME_REQUIRE(A.mode == 0 || (nin == 1 && nout == 1 && !A.reduce), "line pass: FFT-only modes are single-buffer");
+  ME_REQUIRE(A.nbatch >= 1, "line pass: empty batch");
+  int au, av;
+  bool contig;
+  line_layout(g, axis, contig, au, av);
   A.st_a = g.st[axis];
   A.st_u = g.st[au];
   A.st_v = g.st[av];
@@ -363,17 +431,127 @@ int launch_linepass(const VolGeom& g, int axis, int nin, int nout, LineArgs A, c
         ME_REQUIRE(A.coef != nullptr, "line pass: missing coefficient table");
     }
   // grid.x (line tiles) is set per line length by the launcher
-  const long long nblk = contig ? ((long long)A.n_u * A.n_v + kLinesContig - 1) / kLinesContig
-                                : (long long)A.n_u * A.n_v;
+  const long long nblk = contig ? ((long long)A.n_u * A.n_v + 7) / 8 : (long long)A.n_u * A.n_v;
   ME_REQUIRE(nblk > 0 && nblk < (1LL << 31), "line pass: grid too large");
   dim3 grid(1u, A.reduce ? 1u : (unsigned)A.nbatch);
   ME_REQUIRE(A.reduce || A.nbatch <= 65535, "line pass: batch too large");
-  const bool ok = contig ? launch_io<true>(g.n[axis], nin, nout, A, grid, stream)
-                         : launch_io<false>(g.n[axis], nin, nout, A, grid, stream);
+  const int lines = tuned_lines(LineKey{g.n[axis], contig ? 1 : 0, A.n_u, A.n_v, A.st_a, A.st_u, A.st_v});
+  const bool ok = contig ? launch_io<true>(g.n[axis], nin, nout, A, grid, stream, lines)
+                         : launch_io<false>(g.n[axis], nin, nout, A, grid, stream, lines);
   ME_REQUIRE(ok, "line pass: unsupported line length " + std::to_string(g.n[axis]) +
                      " (supported: 8,12,16,24,32,48,64,96,128,192,256,384,512)");
   ME_TRY_HIP(hipGetLastError());
   return 0;
 }
 
+
+static int tune_set(const VolGeom& g, int axis, int lines) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  const LineKey k = line_key(g, axis);
+  if (lines == 0)
+    g_tune.erase(k);
+  else
+    g_tune[k] = lines;
+  return 0;
+}
+
 }  // namespace me
+
+using namespace me;
+
+static bool geom_from(const int* n, const long long* st, VolGeom& g) {
+  if (!n || !st) return false;
+  for (int a = 0; a < 3; ++a) {
+    if (n[a] < 1 || st[a] < 1) return false;
+    g.n[a] = n[a];
+    g.st[a] = st[a];
+    g.r[a] = nullptr;
+  }
+  return true;
+}
+
+extern "C" {
+
+int me_line_tuning_set(const int* n, const long long* st, int axis, int lines) {
+  VolGeom g;
+  ME_REQUIRE(geom_from(n, st, g) && axis >= 0 && axis < 3, "line_tuning_set: bad geometry");
+  bool contig;
+  int au, av;
+  line_layout(g, axis, contig, au, av);
+  ME_REQUIRE(lines == 0 || lines_built(n[axis], contig, lines),
+             "line_tuning_set: lines per workgroup not built for this line length (8, 16 or 32)");
+  return tune_set(g, axis, lines);
+}
+
+int me_line_tuning_get(const int* n, const long long* st, int axis) {
+  VolGeom g;
+  if (!geom_from(n, st, g) || axis < 0 || axis > 2) return 0;
+  return tuned_lines(line_key(g, axis));
+}
+
+void me_line_tuning_clear(void) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.clear();
+}
+
+int me_tune_line_passes(const void* x, void* y, long long bstride, int nbatch, const int* n, const long long* st,
+                        int reps, int* chosen, float* times_ms, hipStream_t stream) {
+  VolGeom g;
+  ME_REQUIRE(x && y && nbatch >= 1 && reps >= 1 && geom_from(n, st, g), "tune_line_passes: bad argument");
+  static const int kCand[3] = {8, kLinesDefault, 32};
+  hipEvent_t e0, e1;
+  ME_TRY_HIP(hipEventCreate(&e0));
+  ME_TRY_HIP(hipEventCreate(&e1));
+  int rc = 0;
+  for (int axis = 0; axis < 3 && rc == 0; ++axis) {
+    bool contig;
+    int au, av;
+    line_layout(g, axis, contig, au, av);
+    LineArgs A{};
+    A.nbatch = nbatch;
+    A.mode = 0;
+    A.in[0] = (const float2*)x;
+    A.in_bs[0] = bstride;
+    A.out[0] = (float2*)y;
+    A.out_bs[0] = bstride;
+    HostPhase ph[2][2];
+    ph[0][0].active = true;  // unit phase: FFT, scale, inverse FFT
+    int best = kLinesDefault;
+    float best_t = 0.f;
+    for (int c = 0; c < 3 && rc == 0; ++c) {
+      const int L = kCand[c];
+      if (times_ms) times_ms[axis * 3 + c] = -1.f;
+      if (!lines_built(n[axis], contig, L)) continue;
+      tune_set(g, axis, L);
+      rc = launch_linepass(g, axis, 1, 1, A, ph, stream);  // warm-up
+      if (rc) break;
+      if (hipEventRecord(e0, stream) != hipSuccess) {
+        set_error("tune_line_passes: hipEventRecord failed");
+        rc = 1;
+      }
+      for (int r = 0; r < reps && rc == 0; ++r) rc = launch_linepass(g, axis, 1, 1, A, ph, stream);
+      float ms = 0.f;
+      if (rc == 0 && (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                      hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) {
+        set_error("tune_line_passes: timing events failed");
+        rc = 1;
+      }
+      if (rc) break;
+      ms /= (float)reps;
+      if (times_ms) times_ms[axis * 3 + c] = ms;
+      if (best_t == 0.f || ms < best_t) {
+        best_t = ms;
+        best = L;
+      }
+    }
+    if (rc == 0) {
+      tune_set(g, axis, best);
+      if (chosen) chosen[axis] = best;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+}  // extern "C"

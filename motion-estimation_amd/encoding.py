@@ -235,10 +235,18 @@ class SenseProblem:
         return torch.zeros(self.vol, dtype=torch.complex64, device=self.device)
 
     # ------------------------------------------------------------ kernels
+    def _coef_ws(self, B):
+        """Coefficient table of a transform call (B rows of 16 doubles); one
+        persistent buffer, so the outer iteration allocates nothing and can
+        be captured into a graph."""
+        if getattr(self, "_coef", None) is None or self._coef.shape[0] < B:
+            self._coef = torch.empty((max(B, self.batch), 16), dtype=torch.float64, device=self.device)
+        return self._coef[:B]
+
     def _rigid(self, x, x_bs, out, theta, mode, flags=0, work=None, mask=None, x_parts=1):
         g = self.geom
         B = theta.shape[0]
-        coef = torch.empty((B, 16), dtype=torch.float64, device=self.device)
+        coef = self._coef_ws(B)
         _lib.call("me_rigid_transform", _lib.ptr(x), x_bs, int(x_parts), _lib.ptr(out), self.vol, _lib.ptr(work),
                   self.vol, int(B),
                   g.n_arg(), g.st_arg(True), *g.r_args(), _lib.ptr(theta), int(theta.stride(0)), _lib.ptr(coef),
@@ -247,7 +255,7 @@ class SenseProblem:
     def _deriv(self, x, out, theta, p_idx, work):
         g = self.geom
         B = theta.shape[0]
-        coef = torch.empty((B, 16), dtype=torch.float64, device=self.device)
+        coef = self._coef_ws(B)
         _lib.call("me_rigid_deriv", _lib.ptr(x), 0, _lib.ptr(out), self.vol, _lib.ptr(work), self.vol, int(B),
                   g.n_arg(), g.st_arg(True), *g.r_args(), _lib.ptr(theta), int(theta.stride(0)), _lib.ptr(coef),
                   int(p_idx), _lib.stream(self.device))
@@ -261,7 +269,7 @@ class SenseProblem:
         if getattr(self, "_chain", None) is None:
             self._chain = torch.empty((2, self.batch, self.vol), dtype=torch.complex64, device=self.device)
         W = self._wbuf()
-        coef = torch.empty((B, 16), dtype=torch.float64, device=self.device)
+        coef = self._coef_ws(B)
         _lib.call("me_rigid_deriv_chain", _lib.ptr(x), 0, _lib.ptr(self._chain[0]), _lib.ptr(self._chain[1]),
                   _lib.ptr(W), self.vol, _lib.ptr(out), self.vol, _lib.ptr(W), self.vol, int(B), g.n_arg(),
                   g.st_arg(True), *g.r_args(), _lib.ptr(theta), int(theta.stride(0)), _lib.ptr(coef), int(p_idx),
@@ -350,9 +358,9 @@ class SenseProblem:
         ``norms0`` (nshots FP64) receives ||E_s T_s x0 - y_s||^2 of the
         initial x from the same first encoding."""
         n = self.vol
-        Ap = torch.empty_like(x)
-        r = torch.empty_like(x)
-        p = torch.empty_like(x)
+        if getattr(self, "_cgv", None) is None:
+            self._cgv = torch.empty((3, n), dtype=torch.complex64, device=self.device)
+        Ap, r, p = self._cgv[0], self._cgv[1], self._cgv[2]
         st = self._cg_state
         s = _lib.stream(self.device)
         self.normal(x, Ap, theta, y=track, norms=norms0)
@@ -389,9 +397,10 @@ class SenseProblem:
         """sum_s ||M_s (F S T_s x - y)||^2  (joint_recon.py:108-115)."""
         return float(self.objective_per_shot(x, theta).sum().item())
 
-    def lm_step(self, x, tol=1e-6, record=None, tracked=False, use_ga=False):
+    def lm_step(self, x, tol=1e-6, record=None, tracked=False, use_ga=False, out=None):
         """One LM update of every shot (transform_estimation.py:204-315).
-        Updates self.theta/damp/flags in place, returns T(mean) x.
+        Updates self.theta/damp/flags in place, returns T(mean) x (written to
+        ``out`` when given, which may be x itself).
 
         ``tracked``: the track buffer already holds E T x for this x and theta
         (from cg(track=...)), so the residual is formed from it instead of a
@@ -439,10 +448,13 @@ class SenseProblem:
         if record is not None:
             record.append(dict(e0=self._stats[:, 0].cpu().numpy().copy(), e1=self._e1.cpu().numpy().copy(),
                                accepted=self._acc.cpu().numpy().astype(bool), delta=self._delta.cpu().numpy().copy()))
-        mean = torch.empty((1, 6), dtype=torch.float64, device=dev)
+        if getattr(self, "_mean", None) is None:
+            self._mean = torch.empty((1, 6), dtype=torch.float64, device=dev)
+        mean = self._mean
         _lib.call("me_lm_mean", _lib.ptr(th), 6, self.nshots, _lib.ptr(mean), s)
-        out = torch.empty(self.vol, dtype=torch.complex64, device=dev)
-        self._rigid(x, 0, out, mean, 0)
+        if out is None:
+            out = torch.empty(self.vol, dtype=torch.complex64, device=dev)
+        self._rigid(x, 0, out, mean, 0)  # in place when out is x
         return out
 
     def _ga_candidates(self, x, s0, s1, eps=1e-4):

@@ -50,12 +50,14 @@ class JointRecon:
         from, i.e. the previous iteration's objective (joint_recon.py:102),
         computed from the CG's first encoding of that same state."""
         pb = self.problem
-        b = pb.rhs()
+        if getattr(self, "_b", None) is None:
+            self._b = pb.zeros_image()
+        b = pb.rhs(out=self._b)
         track = pb.track_buffer() if self.max_nm_iter > 0 else None
         pb.cg(self.x, b, self.max_cg_iter, 1e-6, check_every=self.cg_check_every, track=track,
               norms0=prev_objective)
         for i in range(self.max_nm_iter):
-            self.x = pb.lm_step(self.x, tracked=(i == 0))
+            pb.lm_step(self.x, tracked=(i == 0), out=self.x)
             pb.mask_(self.x)
 
     def objective(self):

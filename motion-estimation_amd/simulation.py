@@ -271,3 +271,38 @@ def make_config(name, device=None, snr_db=30.0, shape=None, coils=None):
     if snr_db is not None:
         ksp = add_noise(ksp, snr_db, A.any(0))
     return dict(gt=gt, mps=mps, ksp=ksp, A=A, A_corr=Acorr, t_corr=tc, config=cfg)
+
+
+BATCH_SHAPES = ((64, 64, 48), (96, 96, 64), (128, 128, 96))
+
+
+def batch_specs(n_volumes=24, seed=4, shapes=BATCH_SHAPES, coils=(8, 20), shots=(16, 64)):
+    """Seeded descriptions of a set of uneven volumes: shape from ``shapes``,
+    coil count uniform in ``coils``, and a DISORDER tile (par, lin) with
+    sides in {4, 8, 16} and acceleration in {1, 2, 4} whose shot count
+    prod(tile) / accel lies in ``shots``; random-walk motion per shot."""
+    rng = np.random.default_rng(seed)
+    combos = [((a, b), r) for a in (4, 8, 16) for b in (4, 8, 16) for r in (1, 2, 4)
+              if shots[0] <= a * b // r <= shots[1] and (a * b) % r == 0]
+    out = []
+    for v in range(n_volumes):
+        shape = tuple(shapes[int(rng.integers(len(shapes)))])
+        tile, accel = combos[int(rng.integers(len(combos)))]
+        out.append(dict(shape=shape, coils=int(rng.integers(coils[0], coils[1] + 1)), tile=tile, accel=accel,
+                        nshots=tile[0] * tile[1] // accel, seed=int(rng.integers(1 << 30))))
+    return out
+
+
+def make_volume(spec, device=None, snr_db=30.0):
+    """Synthetic centred problem of one :func:`batch_specs` entry (motion-
+    corrupted with the same per-shot random walk it is reconstructed with)."""
+    device = device or _lib.require_gpu()
+    shp, sd = spec["shape"], spec["seed"]
+    gt = phantom(shp, seed=sd % 1000, device=device)
+    mps = coil_maps(shp, spec["coils"], seed=sd % 1000 + 1, device=device)
+    A, _ = disorder_sampling(shp, spec["tile"], spec["accel"], nshots=spec["nshots"], seed=sd % 100000)
+    tc = motion_random_walk(spec["nshots"], seed=sd % 1000 + 2)
+    ksp = generate_corrupted_kspace(gt, mps, A, tc)
+    if snr_db is not None:
+        ksp = add_noise(ksp, snr_db, A.any(0), seed=sd % 1000 + 3)
+    return dict(gt=gt, mps=mps, ksp=ksp, A=A, t_corr=tc, spec=spec)
