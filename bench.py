@@ -42,6 +42,11 @@ def parse():
                     help="differentiation workload instead of the headline: data-consistency loss forward+backward "
                          "at c3 (192x192x128, 24 coils, 48 shots), recompute on/off, and a 64-direction central "
                          "difference check at step 1e-3 (prints its own JSON line)")
+    ap.add_argument("--batch", action="store_true",
+                    help="batching workload instead of the headline: 24 seeded volumes of uneven size "
+                         "(64x64x48 / 96x96x64 / 128x128x96), coils 8-20, shots 16-64; one-at-a-time eager outer "
+                         "iterations against graph replays over concurrent streams, before and after the "
+                         "line-pass autotuner (prints its own JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (three times), the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
@@ -223,6 +228,66 @@ def grad_bench(args, torch, me, dev):
     print(json.dumps(out), flush=True)
 
 
+def batch_bench(args, torch, me, dev):
+    """Batched vs one-at-a-time joint reconstruction of uneven volumes."""
+    specs = me.simulation.batch_specs(24, seed=4)
+    vols = []
+    t_setup = time.perf_counter()
+    for i, sp in enumerate(specs):
+        d = me.simulation.make_volume(sp, device=dev)
+        kg, rkg = me.compute_transform_grids_voxel(sp["shape"], (1, 1, 1))
+        vols.append(dict(ksp=d["ksp"], mps=d["mps"], shot_mask=d["A"], kgrid=kg, rkgrid=rkg))
+        del d
+    log(f"generated {len(vols)} volumes in {time.perf_counter() - t_setup:.1f}s")
+    bj = me.BatchedJointRecon(vols, nstreams=4)
+    del vols
+    me._lib.load().me_line_tuning_clear()
+    bj.step_eager()  # allocates every workspace
+    torch.cuda.synchronize()
+    log("warm-up outer iteration done")
+    nvol = len(bj.recons)
+    res = {}
+
+    def timed(name, fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[name] = dict(ms_per_iteration=round(dt * 1e3 / args.steps, 2),
+                         volume_iters_per_s=round(nvol * args.steps / dt, 2))
+        log(f"{name}: {res[name]}")
+
+    timed("one_at_a_time", bj.step_eager)
+    timed("graph_default_lines", bj.step)
+    bj.close()
+    tuning = bj.tune(reps=5)
+    log(f"tuning: {tuning}")
+    timed("graph_tuned", bj.step)
+    bj.close()
+    me._lib.load().me_line_tuning_clear()
+    voxels = sum(int(r.problem.vol) for r in bj.recons)
+    out = {
+        "metric": "batched joint-recon volume outer-iterations/s, 24 uneven volumes",
+        "value": res["graph_tuned"]["volume_iters_per_s"], "unit": "volume_iters/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": "24 volumes, shapes from {64x64x48, 96x96x64, 128x128x96}, coils 8-20, DISORDER "
+                               "shots 16-64, random-walk motion, 3 CG + 1 LM per outer iteration",
+                   "volumes": [dict(shape=list(sp["shape"]), coils=sp["coils"], shots=sp["nshots"],
+                                    tile=list(sp["tile"]), accel=sp["accel"]) for sp in specs],
+                   "total_voxels": voxels, "streams": len(bj.streams)},
+        "legs": res,
+        "speedup_vs_one_at_a_time": round(res["graph_tuned"]["volume_iters_per_s"]
+                                          / res["one_at_a_time"]["volume_iters_per_s"], 3),
+        "tuning": {"x".join(map(str, k)): dict(lines=list(v[0]), ms=list(v[1])) for k, v in tuning.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -242,6 +307,9 @@ def main():
     torch.cuda.set_device(dev)
     if args.grad:
         grad_bench(args, torch, me, dev)
+        return
+    if args.batch:
+        batch_bench(args, torch, me, dev)
         return
     t_setup = time.perf_counter()
     log(f"generating synthetic data for {args.config}")
