@@ -665,8 +665,10 @@ def fold_rx(h):
     return hp.reshape((-1, hp.shape[2]), order="F")
 
 
-def rx_espirit_3d(calib, imsize, kernel_size=(5, 5, 5), eig_thresh=0.02, mask_thresh=0.99, info=None):
-    """espirit.py:47-239 in FP64: calib (NKx, NKy, NKz, NRx) -> maps (Nx, Ny, Nz, NRx)."""
+def rx_espirit_3d(calib, imsize, kernel_size=(5, 5, 5), eig_thresh=0.02, mask_thresh=0.99, info=None, z_slices=None):
+    """espirit.py:47-239 in FP64: calib (NKx, NKy, NKz, NRx) -> maps (Nx, Ny, Nz, NRx).
+    ``z_slices`` (timing samples only) restricts the per-voxel stage to those
+    z indices; the other slices stay zero."""
     calib = np.asarray(calib, dtype=np.complex128)
     NRx = calib.shape[3]
     kx, ky, kz = kernel_size
@@ -691,7 +693,7 @@ def rx_espirit_3d(calib, imsize, kernel_size=(5, 5, 5), eig_thresh=0.02, mask_th
     py0, py1 = int(math.ceil((imsize[1] - ky) / 2)), int(math.floor((imsize[1] - ky) / 2))
     norm_factor = np.prod(imsize) / np.sqrt(np.prod(kernel_size))
     lam_max = np.zeros(imsize)
-    for z in range(imsize[2]):
+    for z in (range(imsize[2]) if z_slices is None else z_slices):
         t = np.pad(Uz[:, :, z], [(px0, px1), (py0, py1), (0, 0), (0, 0)])
         t = np.fft.fftshift(np.fft.ifftn(np.fft.ifftshift(t, axes=(0, 1)), axes=(0, 1)), axes=(0, 1))
         M = t.reshape(-1, NRx, nc)
