@@ -390,6 +390,32 @@ long long me_espirit_maps_workspace_size(int NRx, const int* kernel, const int* 
 int me_espirit_maps(const double* P, int NRx, const int* kernel, const int* imsize, double mask_thresh, int iters,
                     void* work, void* sens, hipStream_t stream);
 
+/* ---------------------------------------------------------------- metrics
+ * Image-quality figures of the experiment scripts (simulation.py:87-97,
+ * 476-477). */
+
+/* Doubles of the partials workspace of me_abs_diff_norm. */
+int me_metric_partials_size(void);
+
+/* out_abs[i] = |a_i - b_i| (complex64 in, float32 out; nullable) and
+ * *sumsq = sum of its squares in FP64: err_volume and err_img^2 of
+ * simulation.py:476-477. */
+int me_abs_diff_norm(const void* a, const void* b, long long n, float* out_abs, double* partials, double* sumsq,
+                     hipStream_t stream);
+
+/* *max_bits = bit pattern of the largest gradient magnitude of vol (float32,
+ * C order, extents n[3]): sqrt(gx^2 + gy^2 + gz^2) with forward differences
+ * and a zero last difference per axis (simulation.py:91-94). */
+int me_gradient_max(const float* vol, const int* n, unsigned int* max_bits, hipStream_t stream);
+
+/* Gradient entropy of simulation.py:87-97: 256-bin histogram of the gradient
+ * magnitude over the float32 bin edges edges[257] (NumPy's linspace over
+ * [0, max + 1e-8]), binned by NumPy's uniform-bin rule (counts[256] receive
+ * the integer counts), density over float32 bin widths, and
+ * *entropy = -sum (h + 1e-12) log2(h + 1e-12) in FP64. */
+int me_gradient_entropy(const float* vol, const int* n, const float* edges, unsigned long long* counts,
+                        double* entropy, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,3 +27,4 @@ from .sense import SenseRecon, sense_recon  # noqa: F401
 from .calibration import coil_compression, noise_whiten  # noqa: F401
 from .espirit import rx_espirit_3d  # noqa: F401
 from .batched import BatchedJointRecon  # noqa: F401
+from . import io, metrics  # noqa: F401

@@ -53,6 +53,9 @@ _SIGS = {
     "me_coil_mix": [vp, i32, i32, vp, i64, vp, vp],
     "me_espirit_calibration_matrix": [vp, vp, vp, vp, vp],
     "me_line_tuning_set": [vp, vp, i32, i32],
+    "me_abs_diff_norm": [vp, vp, i64, vp, vp, vp, vp],
+    "me_gradient_max": [vp, vp, vp, vp],
+    "me_gradient_entropy": [vp, vp, vp, vp, vp, vp],
     "me_tune_line_passes": [vp, vp, i64, i32, vp, vp, i32, vp, vp, vp],
     "me_graph_begin": [vp],
     "me_graph_end": [vp, vp],
@@ -82,6 +85,7 @@ _SIZES = {
     "me_coil_gram_partials_size": ([i32], C.c_longlong),
     "me_espirit_hankel_size": ([vp, vp], C.c_longlong),
     "me_line_tuning_get": ([vp, vp, i32], C.c_int),
+    "me_metric_partials_size": ([], C.c_int),
     "me_jacobi_workspace_size": ([i32], C.c_longlong),
     "me_espirit_maps_workspace_size": ([i32, vp, vp], C.c_longlong),
 }

@@ -55,6 +55,10 @@ rx_espirit_3d                   Hankel structure of a single nonzero sample;
 sense_recon                     full-sampling closed form (A^H A = I: one
                                 step); explicit-matrix solve; zero data
                                 stops before the first step
+gradient_entropy                constant volume and single-axis ramp: counts
+                                and entropy in closed form
+error_volume                    hand-computed |a - b| and norm
+shot_masks_from_counters        hand-built acquisition order, remainder drop
 ==============================  ===============================================
 """
 from __future__ import annotations
@@ -913,3 +917,38 @@ def generate_corrupted_kspace(gt_img, mps, A, transforms, prec="f64"):
         t = rigid_apply(gt, transforms[s], kgrid, rkgrid, prec=prec)
         ksp += (_fftn((t * m).astype(cdt, copy=False), (-3, -2, -1), WORKERS) * a[s]).astype(cdt, copy=False)
     return np.fft.fftshift(ksp, axes=(-3, -2, -1))
+
+
+# ---------------------------------------------------------------------------
+# f4: metrics and shot masks of the real-data scripts
+# (simulation.py:87-97, 476-477; recon.py:158-165)
+# ---------------------------------------------------------------------------
+def gradient_entropy(vol):
+    """simulation.py:87-97: forward differences per axis with the last one
+    zero, gradient magnitude in the volume's dtype, NumPy's 256-bin density
+    histogram over [0, max + 1e-8], entropy of (h + 1e-12) in bits.  Returns
+    (entropy, counts)."""
+    v = np.asarray(vol)
+    g = [np.diff(v, axis=a, append=np.take(v, [-1], axis=a)) for a in range(3)]
+    gmag = np.sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2])
+    counts, edges = np.histogram(gmag, bins=256, range=(0, gmag.max() + 1e-8))
+    dens = counts / np.diff(edges) / counts.sum()
+    h = dens + 1e-12
+    return float(-np.sum(h * np.log2(h))), counts
+
+
+def error_volume(corrected, gt):
+    """simulation.py:476-477: |corrected - gt| and its 2-norm."""
+    e = np.abs(np.asarray(corrected) - np.asarray(gt))
+    return e, float(np.linalg.norm(e))
+
+
+def shot_masks_from_counters(lines, partitions, n_shots, n_par, n_lin):
+    """recon.py:158-165, loop form: shot s marks (partition, line) of samples
+    s*sps .. (s+1)*sps - 1, sps = len(lines) // n_shots."""
+    mask = np.zeros((n_shots, n_par, n_lin), dtype=bool)
+    sps = len(lines) // n_shots
+    for s in range(n_shots):
+        for j in range(s * sps, (s + 1) * sps):
+            mask[s, partitions[j], lines[j]] = True
+    return mask[..., None]

@@ -603,3 +603,49 @@ def test_espirit_recovers_known_maps():
     assert np.median(corr) > 0.999 and corr.min() > 0.98
     assert np.allclose(np.linalg.norm(s, axis=1), 1, atol=1e-9)
     assert np.all(np.abs(s[:, 0].imag) < 1e-9) and np.all(s[:, 0].real >= 0)
+
+
+def test_gradient_entropy_constant_volume():
+    """All gradients zero: every voxel in bin 0 of [0, 1e-8]; density 1/db0."""
+    v = np.full((5, 4, 3), 2.5, dtype=np.float32)
+    ent, counts = O.gradient_entropy(v)
+    assert counts[0] == v.size and counts[1:].sum() == 0
+    hi = np.float32(0) + 1e-8
+    edges = np.linspace(0, hi, 257, dtype=np.result_type(0, hi, np.float32))
+    db = np.diff(edges).astype(np.float64)
+    h = np.full(256, 1e-12)
+    h[0] += 1.0 / db[0]
+    assert np.isclose(ent, -np.sum(h * np.log2(h)), rtol=1e-12)
+
+
+def test_gradient_entropy_ramp():
+    """A ramp of slope 0.5 along axis 1: gradient 0.5 except the last y-plane
+    (0); 0.5 + 1e-8 rounds to 0.5 in float32, so 0.5 falls on the last edge
+    and goes to the last bin."""
+    n0, n1, n2 = 3, 6, 4
+    v = np.broadcast_to((0.5 * np.arange(n1, dtype=np.float32))[None, :, None], (n0, n1, n2)).copy()
+    ent, counts = O.gradient_entropy(v)
+    zero = n0 * n2
+    assert counts[0] == zero and counts[255] == v.size - zero and counts[1:255].sum() == 0
+    db = 0.5 / 256
+    p0, p1 = zero / v.size / db, (v.size - zero) / v.size / db
+    h = np.full(256, 1e-12)
+    h[0] += p0
+    h[255] += p1
+    assert np.isclose(ent, -np.sum(h * np.log2(h)), rtol=1e-6)
+
+
+def test_error_volume_known():
+    a = np.array([3 + 4j, 1 + 0j, 0 + 2j])
+    b = np.array([0 + 0j, 1 + 0j, 0 - 1j])
+    e, n = O.error_volume(a, b)
+    assert np.allclose(e, [5, 0, 3]) and np.isclose(n, np.sqrt(34))
+
+
+def test_shot_masks_from_counters_known():
+    lines = [0, 1, 2, 3, 4, 5, 6]
+    parts = [1, 1, 0, 0, 2, 2, 3]
+    m = O.shot_masks_from_counters(lines, parts, 3, 4, 8)
+    assert m.shape == (3, 4, 8, 1)
+    assert sorted(map(tuple, np.argwhere(m[..., 0]).tolist())) == [(0, 1, 0), (0, 1, 1), (1, 0, 2), (1, 0, 3),
+                                                                    (2, 2, 4), (2, 2, 5)]  # sample 6 dropped
