@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--config", default="c5", help="c1, c2, c3 or c5 (default: the headline workload)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-objective", action="store_true", help="skip the per-iteration objective")
+    ap.add_argument("--no-tune", action="store_true", help="skip the line-pass autotuner at setup")
     ap.add_argument("--grad", action="store_true",
                     help="differentiation workload instead of the headline: data-consistency loss forward+backward "
                          "at c3 (192x192x128, 24 coils, 48 shots), recompute on/off, and a 64-direction central "
@@ -411,6 +412,10 @@ def main():
     app = me.JointRecon(data["ksp"], data["mps"], data["A"], kgrid, rkgrid, max_cg_iter=3, max_nm_iter=1,
                         compute_objective=not args.no_objective)
     del data
+    tuning = None
+    if not args.no_tune:
+        tuning = me.batched.tune_problem(app.problem)
+        log(f"line-pass tuning (lines per axis, ms per candidate): {tuning}")
     torch.cuda.synchronize()
     log(f"setup done in {time.perf_counter() - t_setup:.1f}s; shot batch {app.problem.batch}")
 
@@ -469,7 +474,8 @@ def main():
                                f"{nshots} DISORDER shots, 3 CG + 1 LM per outer iteration"
                                + ("" if not args.no_objective else ", objective skipped"),
                    "shape": list(shape), "coils": ncoils, "shots": nshots, "cg_iters": 3,
-                   "objective": not args.no_objective, "parallelism": "single GPU"},
+                   "objective": not args.no_objective, "parallelism": "single GPU",
+                   "line_tuning": None if tuning is None else list(tuning[0])},
         "roofline": roof,
     }
     if not args.no_cpu_baseline:

@@ -30,6 +30,19 @@ from . import _lib
 from .joint_recon import JointRecon
 
 
+def tune_problem(pb, reps=5):
+    """Run the line-pass autotuner for one problem's geometry (plane-major
+    layout, its shot batch) on its scratch buffer; returns (chosen lines per
+    axis, mean ms per candidate).  Results do not depend on the choice."""
+    n0, n1, n2 = pb.shape
+    U = pb._ubuf()
+    chosen = (C.c_int * 3)()
+    times = (C.c_float * 9)()
+    _lib.call("me_tune_line_passes", _lib.ptr(U), _lib.ptr(U), pb.vol, int(pb.batch), _lib.int3(pb.shape),
+              _lib.ll3((n1, 1, n0 * n1)), int(reps), chosen, times, _lib.stream(pb.device))
+    return tuple(chosen), tuple(round(t, 4) for t in times)
+
+
 class BatchedJointRecon:
     def __init__(self, volumes, max_cg_iter=3, max_nm_iter=1, nstreams=4):
         """volumes: iterable of dicts with keys ksp, mps, shot_mask, kgrid,
@@ -55,16 +68,8 @@ class BatchedJointRecon:
         Returns {shape: (chosen lines per axis, per-candidate ms)}."""
         done = {}
         for r in self.recons:
-            pb = r.problem
-            if pb.shape in done:
-                continue
-            n0, n1, n2 = pb.shape
-            U = pb._ubuf()
-            chosen = (C.c_int * 3)()
-            times = (C.c_float * 9)()
-            _lib.call("me_tune_line_passes", _lib.ptr(U), _lib.ptr(U), pb.vol, int(pb.batch), _lib.int3(pb.shape),
-                      _lib.ll3((n1, 1, n0 * n1)), int(reps), chosen, times, _lib.stream(self.device))
-            done[pb.shape] = (tuple(chosen), tuple(round(t, 4) for t in times))
+            if r.problem.shape not in done:
+                done[r.problem.shape] = tune_problem(r.problem, reps)
         self.tuning = done
         return done
 

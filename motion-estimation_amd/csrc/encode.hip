@@ -137,8 +137,8 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
 
   // rows are streamed with the next row's map/image values prefetched into
   // registers while the current row is transformed (loads are unconditional:
-  // invalid coils read coil 0 and are zeroed by the mask factor)
-  const float cm = cvalid ? 1.f : 0.f;
+  // a workgroup's coils past ncoils read coil 0's map; their accumulators are
+  // never stored and no other coil reads them)
   float2 sv[R1], uv[R1];
   if (t < R2) {
 #pragma unroll
@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
     if (t < R2) {
       float2 a[R1];
 #pragma unroll
-      for (int m1 = 0; m1 < R1; ++m1) a[m1] = c_scale(c_mul(sv[m1], uv[m1]), cm);
+      for (int m1 = 0; m1 < R1; ++m1) a[m1] = c_mul(sv[m1], uv[m1]);
       {  // unconditional (the last row reloads itself), so the loads land in place
         const int ro = min(x0 + 1, n0 - 1) * n1 + t;
 #pragma unroll
