@@ -391,15 +391,10 @@ __global__ void k_espirit_az(const float2* __restrict__ Q, int NRx, int kx, int 
 // whose trace already fails the mask (lambda_max <= trace for a PSD matrix)
 // skips the iteration.
 constexpr int kVoxThreads = 256;
-template <int G2>
-__global__ void __launch_bounds__(kVoxThreads) k_espirit_voxels(const float2* __restrict__ Az, int NRx, int kx,
-                                                                int ky, int Nx, int Ny, int Nz, float thr2k, int iters,
-                                                                float2* __restrict__ sens) {
-  extern __shared__ double2 smem[];
+// B[dx][c2][c] of image row (y, z) into dst (LDS or global)
+__device__ __forceinline__ void espirit_row_table(const float2* __restrict__ Az, int NRx, int kx, int ky, int Ny,
+                                                  int y, int z, float2* dst) {
   const int Dx = 2 * kx - 1, Dy = 2 * ky - 1;
-  double2* vs = smem;                           // kVoxThreads broadcast slots
-  float2* Bs = (float2*)(smem + kVoxThreads);  // Dx * NRx * NRx
-  const int y = blockIdx.x % Ny, z = blockIdx.x / Ny;
   const long long zoff = (long long)z * NRx * NRx * Dx * Dy;
   const int nb = NRx * NRx * Dx;
   for (int e = threadIdx.x; e < nb; e += blockDim.x) {
@@ -409,7 +404,38 @@ __global__ void __launch_bounds__(kVoxThreads) k_espirit_voxels(const float2* __
     const float2* src = Az + zoff + (long long)e * Dy;
     float2 acc = make_float2(0.f, 0.f);
     for (int j = 0; j < Dy; ++j) acc = c_fma(src[j], esp_phase(y, Ny, j - (ky - 1)), acc);
-    Bs[(dxi * NRx + c2) * NRx + c] = acc;
+    dst[(dxi * NRx + c2) * NRx + c] = acc;
+  }
+}
+
+// Row tables of image rows row0 .. row0 + gridDim.x - 1 (row = y + Ny z) into
+// global memory, for coil counts whose table does not fit LDS.
+__global__ void __launch_bounds__(kVoxThreads) k_espirit_rows(const float2* __restrict__ Az, int NRx, int kx, int ky,
+                                                              int Ny, int row0, float2* __restrict__ Bg) {
+  const int row = row0 + blockIdx.x;
+  espirit_row_table(Az, NRx, kx, ky, Ny, row % Ny, row / Ny,
+                    Bg + (long long)blockIdx.x * (2 * kx - 1) * NRx * NRx);
+}
+
+// STAGED: the row table is built in LDS; else it is read from Bg (filled by
+// k_espirit_rows for rows row0 + blockIdx.x).
+template <int G2, bool STAGED>
+__global__ void __launch_bounds__(kVoxThreads) k_espirit_voxels(const float2* __restrict__ Az, int NRx, int kx,
+                                                                int ky, int Nx, int Ny, int Nz, float thr2k, int iters,
+                                                                const float2* __restrict__ Bg, int row0,
+                                                                float2* __restrict__ sens) {
+  extern __shared__ double2 smem[];
+  const int Dx = 2 * kx - 1;
+  double2* vs = smem;  // kVoxThreads broadcast slots
+  const int rowid = row0 + blockIdx.x;
+  const int y = rowid % Ny, z = rowid / Ny;
+  const float2* Bs;
+  if (STAGED) {
+    float2* Bl = (float2*)(smem + kVoxThreads);  // Dx * NRx * NRx
+    espirit_row_table(Az, NRx, kx, ky, Ny, y, z, Bl);
+    Bs = Bl;
+  } else {
+    Bs = Bg + (long long)blockIdx.x * Dx * NRx * NRx;
   }
   __syncthreads();
   const int lane = threadIdx.x % G2;
@@ -519,13 +545,31 @@ __global__ void __launch_bounds__(kVoxThreads) k_espirit_voxels(const float2* __
   }
 }
 
+constexpr int kRowsPerChunk = 512;  // row tables held in global memory at once
+
 template <int G2>
-int launch_voxels(const float2* Az, int NRx, int kx, int ky, const int* imsize, float thr2k, int iters, float2* sens,
-                  size_t shmem, hipStream_t stream) {
-  ME_TRY_HIP(hipFuncSetAttribute((const void*)k_espirit_voxels<G2>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)shmem));
-  hipLaunchKernelGGL(k_espirit_voxels<G2>, dim3((unsigned)(imsize[1] * imsize[2])), dim3(kVoxThreads), shmem, stream,
-                     Az, NRx, kx, ky, imsize[0], imsize[1], imsize[2], thr2k, iters, sens);
+int launch_voxels(const float2* Az, int NRx, int kx, int ky, const int* imsize, float thr2k, int iters, float2* Bg,
+                  float2* sens, hipStream_t stream) {
+  const long long rows = (long long)imsize[1] * imsize[2];
+  const size_t table = (size_t)NRx * NRx * (2 * kx - 1) * sizeof(float2);
+  const size_t slots = kVoxThreads * sizeof(double2);
+  if (table + slots <= 160 * 1024) {
+    const size_t shmem = table + slots;
+    ME_TRY_HIP(hipFuncSetAttribute((const void*)k_espirit_voxels<G2, true>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    hipLaunchKernelGGL((k_espirit_voxels<G2, true>), dim3((unsigned)rows), dim3(kVoxThreads), shmem, stream, Az, NRx,
+                       kx, ky, imsize[0], imsize[1], imsize[2], thr2k, iters, (const float2*)nullptr, 0, sens);
+  } else {
+    ME_REQUIRE(Bg, "espirit_maps: workspace lacks the row tables for this coil count");
+    for (long long r0 = 0; r0 < rows; r0 += kRowsPerChunk) {
+      const int nr = (int)(rows - r0 < kRowsPerChunk ? rows - r0 : kRowsPerChunk);
+      hipLaunchKernelGGL(k_espirit_rows, dim3((unsigned)nr), dim3(kVoxThreads), 0, stream, Az, NRx, kx, ky,
+                         imsize[1], (int)r0, Bg);
+      hipLaunchKernelGGL((k_espirit_voxels<G2, false>), dim3((unsigned)nr), dim3(kVoxThreads), slots, stream, Az,
+                         NRx, kx, ky, imsize[0], imsize[1], imsize[2], thr2k, iters, (const float2*)Bg, (int)r0,
+                         sens);
+    }
+  }
   ME_TRY_HIP(hipGetLastError());
   return 0;
 }
@@ -637,7 +681,10 @@ int me_espirit_projector(const double* V, int m, const int* idx, int r, void* Xw
 long long me_espirit_maps_workspace_size(int NRx, const int* kernel, const int* imsize) {
   if (!kernel || !imsize || NRx < 1) return 0;
   const long long dxy = (long long)(2 * kernel[0] - 1) * (2 * kernel[1] - 1);
-  return (long long)NRx * NRx * dxy * ((2 * kernel[2] - 1) + imsize[2]);  // complex64 elements
+  long long n = (long long)NRx * NRx * dxy * ((2 * kernel[2] - 1) + imsize[2]);  // complex64 elements: Q, Az
+  const long long table = (long long)NRx * NRx * (2 * kernel[0] - 1);
+  if (table * 8 + kVoxThreads * 16 > 160 * 1024) n += table * kRowsPerChunk;  // row tables in global memory
+  return n;
 }
 
 int me_espirit_maps(const double* P, int NRx, const int* kernel, const int* imsize, double mask_thresh, int iters,
@@ -655,16 +702,15 @@ int me_espirit_maps(const double* P, int NRx, const int* kernel, const int* imsi
   hipLaunchKernelGGL(k_espirit_az, dim3(grid_for(naz)), dim3(256), 0, stream, (const float2*)Q, NRx, kx, ky, kz,
                      imsize[2], Az);
   ME_TRY_HIP(hipGetLastError());
-  const size_t shmem = (size_t)NRx * NRx * (2 * kx - 1) * sizeof(float2) + kVoxThreads * sizeof(double2);
-  ME_REQUIRE(shmem <= 160 * 1024, "espirit_maps: per-row coil matrix table exceeds LDS");
+  float2* Bg = Az + naz;  // row tables, used when they exceed LDS
   ME_REQUIRE((long long)imsize[1] * imsize[2] < (1LL << 31), "espirit_maps: image too large");
   // mask: sqrt(lambda(Cs)) / sqrt(prod kernel) > mask_thresh
   // (a negative threshold passes every voxel with a nonzero principal vector)
   const float thr2k = mask_thresh < 0.0 ? -1.f : (float)(mask_thresh * mask_thresh * (double)kx * ky * kz);
-  if (NRx <= 8) return launch_voxels<8>(Az, NRx, kx, ky, imsize, thr2k, iters, (float2*)sens, shmem, stream);
-  if (NRx <= 16) return launch_voxels<16>(Az, NRx, kx, ky, imsize, thr2k, iters, (float2*)sens, shmem, stream);
-  if (NRx <= 32) return launch_voxels<32>(Az, NRx, kx, ky, imsize, thr2k, iters, (float2*)sens, shmem, stream);
-  return launch_voxels<64>(Az, NRx, kx, ky, imsize, thr2k, iters, (float2*)sens, shmem, stream);
+  if (NRx <= 8) return launch_voxels<8>(Az, NRx, kx, ky, imsize, thr2k, iters, Bg, (float2*)sens, stream);
+  if (NRx <= 16) return launch_voxels<16>(Az, NRx, kx, ky, imsize, thr2k, iters, Bg, (float2*)sens, stream);
+  if (NRx <= 32) return launch_voxels<32>(Az, NRx, kx, ky, imsize, thr2k, iters, Bg, (float2*)sens, stream);
+  return launch_voxels<64>(Az, NRx, kx, ky, imsize, thr2k, iters, Bg, (float2*)sens, stream);
 }
 
 }  // extern "C"

@@ -130,6 +130,20 @@ def test_espirit_matches_oracle(me, imsize, nc, cal, kernel, eig_thresh, mask_th
     _compare_maps(sens, ref, info, mask_thresh)
 
 
+def test_espirit_many_coils_row_tables_in_global_memory(me):
+    """64 coils with a 3x3x3 kernel: the per-row coil-matrix table
+    (64 * 64 * 5 complex64) exceeds LDS, so the tables are built in global
+    memory in chunks of rows; same parity bar."""
+    imsize, nc = (20, 16, 12), 64
+    calib, _, _ = espirit_data(imsize, nc=nc, cal=(10, 10, 8), seed=77)
+    calib = calib.astype(np.complex64)
+    info = {}
+    ref = O.rx_espirit_3d(calib, imsize, kernel_size=(3, 3, 3), info=info)
+    sens = me.rx_espirit_3d(calib, imsize, kernel_size=(3, 3, 3))
+    assert me.rx_espirit_3d.kept == info["kept"]
+    _compare_maps(sens, ref, info, 0.99)
+
+
 def test_espirit_nothing_kept_gives_zero_maps(me):
     calib = np.zeros((8, 8, 8, 3), dtype=np.complex64)
     sens = me.rx_espirit_3d(calib, (10, 10, 10), kernel_size=(3, 3, 3))
