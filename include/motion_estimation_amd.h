@@ -372,11 +372,24 @@ long long me_jacobi_workspace_size(int m);
 int me_jacobi_eig(double* G, double* V, int m, int max_sweeps, double tol, double* eigvals, double* work,
                   int* sweeps_done, hipStream_t stream);
 
-/* P = V_r V_r^H for the r columns idx[0..r) (device ints) of V (m x m
- * interleaved FP64): the projector onto the kept singular subspace of
- * espirit.py:118-137.  Xwork holds m x r complex64. */
-int me_espirit_projector(const double* V, int m, const int* idx, int r, void* Xwork, double* P,
+/* P = V_r V_r^H for the r columns idx[0..r) (device ints) of V (m rows of
+ * ldv interleaved FP64 complex): the projector onto the kept singular
+ * subspace of espirit.py:118-137.  Xwork holds m x r complex64. */
+int me_espirit_projector(const double* V, int m, int ldv, const int* idx, int r, void* Xwork, double* P,
                          hipStream_t stream);
+
+/* C (m x n) = op(A) op(B) in FP64 complex (interleaved, row-major, contiguous):
+ * op 0 uses the matrix as stored (A m x k, B k x n), op 1 its conjugate
+ * transpose (A stored k x m, B stored n x k).  Subspace iteration for
+ * ESPIRiT's kept subspace. */
+int me_zgemm(int opa, int opb, int m, int n, int k, const double* A, const double* B, double* C, hipStream_t stream);
+
+/* n complex FP64 standard normal values (real and imaginary parts N(0,1))
+ * from a counter-based hash of (seed, index): reproducible start blocks. */
+int me_random_normal(double* x, long long n, unsigned long long seed, hipStream_t stream);
+
+/* Column j of X (rows x cols FP64 complex, row-major) times s[j]. */
+int me_scale_columns(double* X, long long rows, int cols, const double* s, hipStream_t stream);
 
 /* Complex64 elements of the workspace of me_espirit_maps. */
 long long me_espirit_maps_workspace_size(int NRx, const int* kernel, const int* imsize);

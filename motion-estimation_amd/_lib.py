@@ -64,7 +64,10 @@ _SIGS = {
     "me_graph_destroy": [vp],
     "me_gram_rows": [vp, i32, i64, vp, vp],
     "me_jacobi_eig": [vp, vp, i32, i32, C.c_double, vp, vp, vp, vp],
-    "me_espirit_projector": [vp, i32, vp, i32, vp, vp, vp],
+    "me_espirit_projector": [vp, i32, i32, vp, i32, vp, vp, vp],
+    "me_zgemm": [i32, i32, i32, i32, i32, vp, vp, vp, vp],
+    "me_random_normal": [vp, i64, C.c_ulonglong, vp],
+    "me_scale_columns": [vp, i64, i32, vp, vp],
     "me_espirit_maps": [vp, i32, vp, vp, C.c_double, i32, vp, vp, vp],
     "me_rigid_deriv_apply": [vp, i64, vp, i64, vp, vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp,
                              i32, i32, vp],

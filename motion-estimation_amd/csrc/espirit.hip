@@ -317,14 +317,14 @@ __global__ void k_diag(const double* __restrict__ G, int m, double* __restrict__
   if (i < m) lam[i] = G[((long long)i * m + i) * 2];
 }
 
-// X (m x r complex64) = columns idx[0..r) of V (m x m FP64 complex)
-__global__ void k_gather_columns(const double* __restrict__ V, int m, const int* __restrict__ idx, int r,
+// X (m x r complex64) = columns idx[0..r) of V (m rows of ldv FP64 complex)
+__global__ void k_gather_columns(const double* __restrict__ V, int m, int ldv, const int* __restrict__ idx, int r,
                                  float2* __restrict__ X) {
   const long long n = (long long)m * r;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (long long)gridDim.x * blockDim.x) {
     const int row = (int)(e / r), j = (int)(e % r);
-    const long long s = ((long long)row * m + idx[j]) * 2;
+    const long long s = ((long long)row * ldv + idx[j]) * 2;
     X[e] = make_float2((float)V[s], (float)V[s + 1]);
   }
 }
@@ -580,6 +580,108 @@ inline unsigned grid_for(long long n, int threads = 256, long long cap = 65536) 
   return (unsigned)(b < cap ? b : cap);
 }
 
+
+// ------------------------------------------------------------ subspace tools
+// C (m x n) = op(A) op(B), FP64 complex interleaved, row-major; op 0: as
+// stored, op 1: conjugate transpose (A stored k x m, B stored n x k).
+// 32 x 32 output tiles, 16-deep k tiles in LDS, 2 x 2 outputs per thread.
+constexpr int kZT = 32, kZK = 16;
+__global__ void __launch_bounds__(256) k_zgemm(int opa, int opb, int m, int n, int k, const double2* __restrict__ A,
+                                               const double2* __restrict__ B, double2* __restrict__ Cm) {
+  __shared__ double2 As[kZK][kZT + 1];
+  __shared__ double2 Bs[kZK][kZT + 1];
+  const int i0 = blockIdx.y * kZT, j0 = blockIdx.x * kZT;
+  const int tid = threadIdx.x, ty = tid / 16, tx = tid % 16;
+  double cr[2][2] = {{0, 0}, {0, 0}}, ci[2][2] = {{0, 0}, {0, 0}};
+  for (int k0 = 0; k0 < k; k0 += kZK) {
+    __syncthreads();
+    for (int e = tid; e < kZT * kZK; e += 256) {
+      int i, kk;
+      double2 v = make_double2(0.0, 0.0);
+      if (opa == 0) {  // A[i][kk], contiguous in kk
+        i = e / kZK;
+        kk = e % kZK;
+        if (i0 + i < m && k0 + kk < k) v = A[(long long)(i0 + i) * k + k0 + kk];
+      } else {  // conj(A[kk][i]), contiguous in i
+        kk = e / kZT;
+        i = e % kZT;
+        if (i0 + i < m && k0 + kk < k) {
+          v = A[(long long)(k0 + kk) * m + i0 + i];
+          v.y = -v.y;
+        }
+      }
+      As[kk][i] = v;
+      int j;
+      double2 w = make_double2(0.0, 0.0);
+      if (opb == 0) {  // B[kk][j], contiguous in j
+        kk = e / kZT;
+        j = e % kZT;
+        if (j0 + j < n && k0 + kk < k) w = B[(long long)(k0 + kk) * n + j0 + j];
+      } else {  // conj(B[j][kk]), contiguous in kk
+        j = e / kZK;
+        kk = e % kZK;
+        if (j0 + j < n && k0 + kk < k) {
+          w = B[(long long)(j0 + j) * k + k0 + kk];
+          w.y = -w.y;
+        }
+      }
+      Bs[kk][j] = w;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < kZK; ++kk) {
+      const double2 a0 = As[kk][2 * ty], a1 = As[kk][2 * ty + 1];
+      const double2 b0 = Bs[kk][2 * tx], b1 = Bs[kk][2 * tx + 1];
+      const double2 av[2] = {a0, a1}, bv[2] = {b0, b1};
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          cr[p][q] = fma(av[p].x, bv[q].x, fma(-av[p].y, bv[q].y, cr[p][q]));
+          ci[p][q] = fma(av[p].x, bv[q].y, fma(av[p].y, bv[q].x, ci[p][q]));
+        }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int i = i0 + 2 * ty + p, j = j0 + 2 * tx + q;
+      if (i < m && j < n) Cm[(long long)i * n + j] = make_double2(cr[p][q], ci[p][q]);
+    }
+}
+
+// x[i] = complex standard normal from a counter-based hash of (seed, i):
+// splitmix64 words, Box-Muller in FP64
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__global__ void k_random_normal(double2* __restrict__ x, long long n, unsigned long long seed) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const unsigned long long a = splitmix64(seed ^ (2ull * (unsigned long long)i));
+    const unsigned long long b = splitmix64(seed ^ (2ull * (unsigned long long)i + 1ull));
+    const double u1 = ((double)(a >> 11) + 0.5) * (1.0 / 9007199254740992.0);  // (0, 1)
+    const double u2 = (double)(b >> 11) * (1.0 / 9007199254740992.0);
+    const double r = sqrt(-2.0 * log(u1));
+    double sn, cs;
+    sincos(2.0 * kPi * u2, &sn, &cs);
+    x[i] = make_double2(r * cs, r * sn);
+  }
+}
+
+// X (rows x cols) *= diag(s) from the right: column j scaled by s[j]
+__global__ void k_scale_columns(double2* __restrict__ X, long long rows, int cols, const double* __restrict__ s) {
+  const long long n = rows * cols;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+    const double f = s[e % cols];
+    X[e].x *= f;
+    X[e].y *= f;
+  }
+}
+
 }  // namespace me
 
 using namespace me;
@@ -670,9 +772,35 @@ int me_jacobi_eig(double* G, double* V, int m, int max_sweeps, double tol, doubl
   return 0;
 }
 
-int me_espirit_projector(const double* V, int m, const int* idx, int r, void* Xwork, double* P, hipStream_t stream) {
-  ME_REQUIRE(V && idx && Xwork && P && m >= 1 && r >= 1 && r <= m, "espirit_projector: bad argument");
-  hipLaunchKernelGGL(k_gather_columns, dim3(grid_for((long long)m * r)), dim3(256), 0, stream, V, m, idx, r,
+int me_zgemm(int opa, int opb, int m, int n, int k, const double* A, const double* B, double* C, hipStream_t stream) {
+  ME_REQUIRE(A && B && C && m >= 1 && n >= 1 && k >= 1, "zgemm: bad argument");
+  ME_REQUIRE((opa == 0 || opa == 1) && (opb == 0 || opb == 1), "zgemm: op must be 0 (as stored) or 1 (conj transpose)");
+  ME_REQUIRE((m + kZT - 1) / kZT < 65536, "zgemm: too many rows");
+  dim3 grid((unsigned)((n + kZT - 1) / kZT), (unsigned)((m + kZT - 1) / kZT));
+  hipLaunchKernelGGL(k_zgemm, grid, dim3(256), 0, stream, opa, opb, m, n, k, (const double2*)A, (const double2*)B,
+                     (double2*)C);
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
+}
+
+int me_random_normal(double* x, long long n, unsigned long long seed, hipStream_t stream) {
+  ME_REQUIRE(x && n >= 1, "random_normal: bad argument");
+  hipLaunchKernelGGL(k_random_normal, dim3(grid_for(n)), dim3(256), 0, stream, (double2*)x, n, seed);
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
+}
+
+int me_scale_columns(double* X, long long rows, int cols, const double* s, hipStream_t stream) {
+  ME_REQUIRE(X && s && rows >= 1 && cols >= 1, "scale_columns: bad argument");
+  hipLaunchKernelGGL(k_scale_columns, dim3(grid_for(rows * cols)), dim3(256), 0, stream, (double2*)X, rows, cols, s);
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
+}
+
+int me_espirit_projector(const double* V, int m, int ldv, const int* idx, int r, void* Xwork, double* P,
+                         hipStream_t stream) {
+  ME_REQUIRE(V && idx && Xwork && P && m >= 1 && r >= 1 && r <= ldv, "espirit_projector: bad argument");
+  hipLaunchKernelGGL(k_gather_columns, dim3(grid_for((long long)m * r)), dim3(256), 0, stream, V, m, ldv, idx, r,
                      (float2*)Xwork);
   ME_TRY_HIP(hipGetLastError());
   return me_gram_rows(Xwork, m, r, P, stream);

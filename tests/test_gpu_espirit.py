@@ -130,17 +130,54 @@ def test_espirit_matches_oracle(me, imsize, nc, cal, kernel, eig_thresh, mask_th
     _compare_maps(sens, ref, info, mask_thresh)
 
 
-def test_espirit_many_coils_row_tables_in_global_memory(me):
-    """64 coils with a 3x3x3 kernel: the per-row coil-matrix table
-    (64 * 64 * 5 complex64) exceeds LDS, so the tables are built in global
-    memory in chunks of rows; same parity bar."""
+@pytest.mark.parametrize("opa,opb,m,n,k", [(0, 0, 37, 45, 29), (1, 0, 33, 17, 70), (0, 1, 5, 64, 40),
+                                            (1, 1, 64, 31, 33)])
+def test_zgemm_matches_numpy(me, opa, opb, m, n, k):
+    rng = np.random.default_rng(m + n + k)
+    a_shape = (m, k) if opa == 0 else (k, m)
+    b_shape = (k, n) if opb == 0 else (n, k)
+    A = rng.standard_normal(a_shape) + 1j * rng.standard_normal(a_shape)
+    B = rng.standard_normal(b_shape) + 1j * rng.standard_normal(b_shape)
+    Ad = torch.view_as_real(torch.as_tensor(A, device=DEV)).contiguous()
+    Bd = torch.view_as_real(torch.as_tensor(B, device=DEV)).contiguous()
+    Cd = torch.empty((m, n, 2), dtype=torch.float64, device=DEV)
+    me._lib.call("me_zgemm", opa, opb, m, n, k, me._lib.ptr(Ad), me._lib.ptr(Bd), me._lib.ptr(Cd),
+                 me._lib.stream(DEV))
+    ref = (A if opa == 0 else A.conj().T) @ (B if opb == 0 else B.conj().T)
+    got = torch.view_as_complex(Cd).cpu().numpy()
+    assert np.linalg.norm(got - ref) <= 1e-14 * np.linalg.norm(ref) * np.sqrt(k)
+
+
+def test_random_normal_moments(me):
+    n = 1 << 18
+    x = torch.empty((n, 2), dtype=torch.float64, device=DEV)
+    me._lib.call("me_random_normal", me._lib.ptr(x), n, 12345, me._lib.stream(DEV))
+    v = x.cpu().numpy()
+    assert abs(v.mean()) < 0.01 and abs(v.std() - 1.0) < 0.01
+    y = torch.empty_like(x)
+    me._lib.call("me_random_normal", me._lib.ptr(y), n, 12345, me._lib.stream(DEV))
+    assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("method", ["jacobi", "subspace"])
+def test_espirit_many_coils_both_routes(me, method):
+    """64 coils with a 3x3x3 kernel (m = 1728 rows): the kept subspace by full
+    Jacobi and by subspace iteration with Rayleigh-Ritz, and the per-row
+    coil-matrix table (64 * 64 * 5 complex64) beyond LDS, so the tables are
+    built in global memory in chunks of rows; same parity bar either way."""
     imsize, nc = (20, 16, 12), 64
     calib, _, _ = espirit_data(imsize, nc=nc, cal=(10, 10, 8), seed=77)
     calib = calib.astype(np.complex64)
     info = {}
     ref = O.rx_espirit_3d(calib, imsize, kernel_size=(3, 3, 3), info=info)
-    sens = me.rx_espirit_3d(calib, imsize, kernel_size=(3, 3, 3))
+    sens = me.rx_espirit_3d(calib, imsize, kernel_size=(3, 3, 3), method=method)
+    assert me.rx_espirit_3d.method == method
     assert me.rx_espirit_3d.kept == info["kept"]
+    S, sv = info["singular_values"], me.rx_espirit_3d.singular_values
+    # the Ritz values are accurate where the block has converged: down to
+    # and past the keep threshold 0.02 * s0
+    kk = int(np.sum(S > 0.01 * S[0]))
+    assert np.max(np.abs(sv[:kk] - S[:kk])) <= 1e-6 * S[0]
     _compare_maps(sens, ref, info, 0.99)
 
 
