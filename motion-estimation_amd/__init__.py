@@ -8,7 +8,13 @@ include/motion_estimation_amd.h):
 * image_estimation: estimate_image_cg (CG-SENSE)
 * transform_estimation: estimate_transform (per-shot Levenberg-Marquardt)
 * joint_recon: JointRecon
-* pyramid: pyramid_reconstruction
+* pyramid: pyramid_reconstruction, aggregate_sampling_masks, aggregate_transforms
+* sense: SenseRecon (the uncorrected SENSE baseline)
+* calibration: noise_whiten, coil_compression; espirit: rx_espirit_3d
+* io: shot masks from scanner counters, the real-data calibration chain,
+  NIfTI-1 and npz outputs; metrics: error volume, gradient entropy
+* autograd: DataConsistencyLoss (gradients w.r.t. image and motion)
+* batched: BatchedJointRecon (many volumes, HIP graph replay, line-pass tuning)
 * simulation: DISORDER sampling, corrupted k-space, synthetic inputs
 
 The package directory is named ``motion-estimation_amd``; load it with
