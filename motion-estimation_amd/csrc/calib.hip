@@ -277,22 +277,31 @@ __global__ void __launch_bounds__(64) k_herm_eig(const double* __restrict__ Ain,
 // y[o][v] = sum_c M[o][c] x[c][v]  (M: cout x cin complex, row-major).
 // Outputs in register blocks of 8; each block re-reads the voxel's inputs
 // (cache hits after the first block).
+// y[o](v) = sum_c M[o][c] x[c](v): each thread reads its voxel's cin inputs
+// once into registers (MAXC >= cin), then forms the outputs 8 at a time from
+// the LDS copy of M, so x is read once and y written once (2 C V bytes).
+template <int MAXC>
 __global__ void __launch_bounds__(256) k_coil_mix(const float2* __restrict__ M, int cout, int cin,
                                                   const float2* __restrict__ x, long long n, float2* __restrict__ y) {
   __shared__ float2 m[kMaxCalibCoils * kMaxCalibCoils];
   for (int e = threadIdx.x; e < cout * cin; e += blockDim.x) m[e] = M[e];
   __syncthreads();
   for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long long)gridDim.x * blockDim.x) {
+    float2 xin[MAXC];
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) xin[c] = c < cin ? x[(long long)c * n + v] : make_float2(0.f, 0.f);
     for (int o0 = 0; o0 < cout; o0 += 8) {
       float2 acc[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) acc[j] = make_float2(0.f, 0.f);
-      for (int c = 0; c < cin; ++c) {
-        const float2 xc = x[(long long)c * n + v];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int o = o0 + j < cout ? o0 + j : cout - 1;
-          acc[j] = c_fma(m[o * cin + c], xc, acc[j]);
+      for (int c = 0; c < MAXC; ++c) {
+        if (c < cin) {
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int o = o0 + j < cout ? o0 + j : cout - 1;
+            acc[j] = c_fma(m[o * cin + c], xin[c], acc[j]);
+          }
         }
       }
 #pragma unroll
@@ -349,8 +358,16 @@ int me_coil_mix(const void* M, int cout, int cin, const void* x, long long n, vo
   ME_REQUIRE(cin >= 1 && cin <= kMaxCalibCoils && cout >= 1 && cout <= kMaxCalibCoils, "coil_mix: 1 to 64 coils");
   ME_REQUIRE(x != y, "coil_mix: output must not alias the input");
   const long long blocks = (n + 255) / 256;
-  hipLaunchKernelGGL(k_coil_mix, dim3((unsigned)(blocks < 8192 ? (blocks > 0 ? blocks : 1) : 8192)), dim3(256), 0,
-                     stream, (const float2*)M, cout, cin, (const float2*)x, n, (float2*)y);
+  const dim3 grid((unsigned)(blocks < 8192 ? (blocks > 0 ? blocks : 1) : 8192));
+  if (cin <= 16)
+    hipLaunchKernelGGL(k_coil_mix<16>, grid, dim3(256), 0, stream, (const float2*)M, cout, cin, (const float2*)x, n,
+                       (float2*)y);
+  else if (cin <= 32)
+    hipLaunchKernelGGL(k_coil_mix<32>, grid, dim3(256), 0, stream, (const float2*)M, cout, cin, (const float2*)x, n,
+                       (float2*)y);
+  else
+    hipLaunchKernelGGL(k_coil_mix<64>, grid, dim3(256), 0, stream, (const float2*)M, cout, cin, (const float2*)x, n,
+                       (float2*)y);
   ME_TRY_HIP(hipGetLastError());
   return 0;
 }
