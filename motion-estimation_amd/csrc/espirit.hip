@@ -385,7 +385,9 @@ __global__ void k_espirit_az(const float2* __restrict__ Q, int NRx, int kx, int 
 // One workgroup per image row (y, z).  B[dx][c2][c] = sum_dy Az[z][c][c2][dx][dy]
 // exp(i theta_y dy) is built in LDS (contiguous in c); then a group of G2
 // lanes (G2 >= NRx) per voxel, lane c holding row c of Cs(v) (FP32).  Power
-// iteration in FP64 from the column of the largest diagonal entry until the
+// iteration in FP64, started from the previous voxel's eigenvector of the
+// same lane group when that voxel was kept (else from the column of the
+// largest diagonal entry), until the
 // estimated distance to the limit, |dv| rho / (1 - rho) with rho the ratio
 // of successive update norms, is below 1e-10 (or iters steps); a voxel
 // whose trace already fails the mask (lambda_max <= trace for a PSD matrix)
@@ -444,6 +446,13 @@ __global__ void __launch_bounds__(kVoxThreads) k_espirit_voxels(const float2* __
   const bool act = lane < NRx;
   const int c = act ? lane : 0;
   double2* vg = vs + grp * G2;
+  // lane groups interleave along the row (balanced across wavefronts); inside
+  // the mask the maps are smooth, so a kept voxel's eigenvector starts the
+  // group's next voxel (ngrp positions on) close to its limit, which does
+  // not depend on the start; after a masked-out voxel the start is the
+  // largest-diagonal column again
+  double pvx = 0.0, pvy = 0.0;
+  bool have_prev = false;  // group-uniform
   for (int x = grp; x < Nx; x += ngrp) {
     float2 row[G2];
 #pragma unroll
@@ -487,6 +496,10 @@ __global__ void __launch_bounds__(kVoxThreads) k_espirit_voxels(const float2* __
           vx = row[c2].x;
           vy = row[c2].y;
         }
+      if (have_prev) {
+        vx = act ? pvx : 0.0;
+        vy = act ? pvy : 0.0;
+      }
       double nrm = vx * vx + vy * vy;
 #pragma unroll
       for (int o = G2 / 2; o > 0; o >>= 1) nrm += __shfl_xor(nrm, o, G2);
@@ -539,7 +552,11 @@ __global__ void __launch_bounds__(kVoxThreads) k_espirit_voxels(const float2* __
         if (dif * rho <= 1e-10 * (1.0 - rho) || dif <= 1e-14) break;  // group-uniform
       }
       // lam: Rayleigh quotient of the last iterate, lambda_max(Cs)
-      if (lam > (double)thr2k) outv = make_float2((float)vx, (float)vy);
+      const bool valid = lam > (double)thr2k;  // group-uniform
+      if (valid) outv = make_float2((float)vx, (float)vy);
+      pvx = vx;
+      pvy = vy;
+      have_prev = valid;
     }
     if (act) sens[(((long long)x * Ny + y) * Nz + z) * NRx + lane] = outv;
   }
