@@ -85,7 +85,10 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
   constexpr int R1 = RowT<NR>::R1, R2 = RowT<NR>::R2, T = RowT<NR>::T, NT = RowT<NR>::NT;
   constexpr int G = kCoilsPerWG;
   __shared__ float2 xch[G * R1 * (R2 + 1)];
-  __shared__ float2 zrow[G * NR];
+  // row spectrum, coil-fastest per column k1 (stride ZS = G + 2 keeps 16-byte
+  // alignment for paired coil reads and spreads the step-2 writes over banks)
+  constexpr int ZS = G + 2;
+  __shared__ __align__(16) float2 zrow[NR * ZS];
   __shared__ float2 twr[NR];
   __shared__ float2 twc[kMaxCol];
   __shared__ double red[16];
@@ -177,7 +180,7 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
       for (int m2 = 0; m2 < R2; ++m2) X[m2] = xch[lidx(t, m2)];
       RDFT<R2, -1>::run(X);
 #pragma unroll
-      for (int k2 = 0; k2 < R2; ++k2) zrow[g * NR + t + R1 * k2] = X[k2];
+      for (int k2 = 0; k2 < R2; ++k2) zrow[(t + R1 * k2) * ZS + g] = X[k2];
     }
     __syncthreads();
 #pragma unroll
@@ -185,7 +188,11 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
       if (mv[j]) {
         const float2 w = twc[tix[j]];
 #pragma unroll
-        for (int q = 0; q < G; ++q) acc[j][q] = c_fma(w, zrow[q * NR + mk1[j]], acc[j][q]);
+        for (int q = 0; q < G; q += 2) {
+          const float4 z = *reinterpret_cast<const float4*>(&zrow[mk1[j] * ZS + q]);
+          acc[j][q] = c_fma(w, make_float2(z.x, z.y), acc[j][q]);
+          acc[j][q + 1] = c_fma(w, make_float2(z.z, z.w), acc[j][q + 1]);
+        }
       }
       tix[j] += mk0[j];
       tix[j] -= (tix[j] >= n0) ? n0 : 0;
