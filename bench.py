@@ -120,6 +120,15 @@ def encode_bytes(pb, s0, s1, meas):
     return float(per.sum())
 
 
+def decode_bytes(pb, s0, s1):
+    """Algorithmic bytes of one me_encode_adjoint launch over shots s0..s1: per
+    shot, the gathered samples (8*C*K_s) and the coil maps (C*V) read once and
+    one image (V) written."""
+    V = pb.vol * 8
+    K = pb.plan.cnt_h[s0:s1] * pb.shape[2]
+    return float((V + pb.ncoils * V + 8 * pb.ncoils * K).sum())
+
+
 def cpu_baseline(shape, ncoils, nshots, n_cg):
     """Times the oracle's complex64 restatement (dense masks, 3-D FFTs) on one
     sample of each unit at the full shape and composes one outer iteration."""
@@ -435,15 +444,16 @@ def main():
         log(f"warmup step {i + 1}/{args.warmup} done")
     pb = app.problem
     timer = KernelTimer(torch)
-    orig_encode = pb._encode
+    orig_encode, orig_decode = pb._encode, pb._decode
     pb._encode = timer.wrap("encode_forward", orig_encode)
+    pb._decode = timer.wrap("encode_adjoint", orig_decode)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    pb._encode = orig_encode
+    pb._encode, pb._decode = orig_encode, orig_decode
     log(f"timed {args.steps} steps in {t1 - t0:.3f}s")
     ms = (t1 - t0) * 1e3 / args.steps
     # dominant kernel: the gathered forward encoding (k_encode_fwd)
@@ -457,6 +467,17 @@ def main():
                 frac=round(achieved / PEAK_HBM_GBS, 4), traffic=read_traffic(),
                 kernel="k_encode_fwd", launches=len(durs), avg_launch_ms=round(1e3 * float(np.mean(durs)), 3),
                 bytes_per_launch=float(np.mean(byts)))
+    # the adjoint encoding (E^H), the second kernel of the normal operator
+    abyts, adurs = [], []
+    for (s, e, a, k), d in zip(timer.events.get("encode_adjoint", []), timer.durations("encode_adjoint")):
+        abyts.append(decode_bytes(pb, a[1], a[2]))
+        adurs.append(d)
+    roof_adj = None
+    if adurs:
+        aach = float(np.sum(abyts) / np.sum(adurs) / 1e9)
+        roof_adj = dict(bound="hbm", achieved=round(aach, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                        frac=round(aach / PEAK_HBM_GBS, 4), kernel="k_encode_adj_wave", launches=len(adurs),
+                        avg_launch_ms=round(1e3 * float(np.mean(adurs)), 3), bytes_per_launch=float(np.mean(abyts)))
     res = {
         "metric": "joint-recon outer iters/s at 256x256x192, 32 coils, 64 shots",
         "value": round(args.steps / (t1 - t0), 4),
@@ -477,6 +498,7 @@ def main():
                    "objective": not args.no_objective, "parallelism": "single GPU",
                    "line_tuning": None if tuning is None else list(tuning[0])},
         "roofline": roof,
+        "roofline_adjoint": roof_adj,
     }
     if not args.no_cpu_baseline:
         log("timing the CPU baseline sample")
