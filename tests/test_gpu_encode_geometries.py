@@ -1,0 +1,89 @@
+"""Encoding parity against the FP64 oracle on geometries that select the other
+kernel variants (image_estimation.py:94-128, transform_estimation.py:207-210):
+
+* column lengths N0 that are not powers of two (the adjoint's twiddle index
+  by multiply-and-correct instead of a mask);
+* rows longer than 256 (N1 = 384, 512: the row-group adjoint for lines of
+  more than 16 threads);
+* shots with more samples per readout plane than one forward workgroup holds
+  (several sample chunks per plane) and more than the adjoint stages at once
+  (512 per coil: several staged chunks).
+
+Tolerance: relative L2 1e-4 on gathered k-space and images."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import motion_oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-4
+
+# (shape, coils, tile, accel, shots): what each case exercises
+CASES = [
+    ((24, 12, 8), 4, (4, 4), 1, 2),    # N0 = 24, 130-134 samples > 128 per forward workgroup
+    ((48, 32, 8), 5, (4, 4), 1, 2),    # N0 = 48, 768 samples: 3 forward chunks, 2 adjoint chunks
+    ((96, 24, 8), 3, (8, 4), 2, 4),    # N0 = 96 = 3 * 32, 279-281 samples: 2 forward chunks
+    ((16, 384, 8), 4, (4, 4), 2, 4),   # N1 = 384: row-group adjoint, power-of-two N0
+    ((12, 512, 8), 3, (4, 4), 2, 4),   # N1 = 512, N0 = 12: row-group adjoint with multiply-and-correct
+]
+
+
+def _rel(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else np.asarray(a)
+    return float(np.linalg.norm(a - np.asarray(b)) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module", params=CASES, ids=lambda c: "x".join(map(str, c[0])))
+def case(request, me):
+    shape, nc, tile, accel, nshots = request.param
+    sim = me.simulation
+    mps = sim.coil_maps(shape, nc, seed=21).numpy().astype(np.complex128)
+    A, _ = sim.disorder_sampling(shape, tile, accel, nshots=nshots)
+    t = sim.motion_uniform(A.shape[0], rot_deg=4.0, trans_vox=1.5, seed=22)
+    rng = np.random.default_rng(23)
+    ksp = rng.standard_normal((nc, *shape)) + 1j * rng.standard_normal((nc, *shape))
+    A_u = np.fft.ifftshift(A, axes=(1, 2))
+    mps_u = np.fft.ifftshift(mps, axes=(-3, -2, -1))
+    kg, rkg = me.compute_transform_grids_voxel(shape, (1, 1, 1))
+    okg, orkg = O.compute_transform_grids_voxel(shape, (1, 1, 1))
+    pb = me.encoding.SenseProblem(ksp, mps_u, A_u, kg, rkg)
+    pb.theta.copy_(torch.as_tensor(t))
+    M = np.sum(np.abs(mps_u) ** 2, axis=0) > 0.1
+    return dict(shape=shape, nc=nc, pb=pb, mps_u=mps_u, A_u=A_u, ksp=ksp, t=t, okg=okg, orkg=orkg, M=M, rng=rng)
+
+
+def test_case_selects_the_intended_paths(case):
+    """The sample counts that make each case cover several chunks."""
+    cnt = int(case["pb"].plan.cnt_h.max())
+    need = {(24, 12, 8): 129, (48, 32, 8): 513, (96, 24, 8): 257, (16, 384, 8): 513, (12, 512, 8): 513}
+    assert cnt >= need[case["shape"]], cnt
+
+
+def test_forward_encoding(case):
+    d = case
+    pb, shape, nc = d["pb"], d["shape"], d["nc"]
+    x = d["rng"].standard_normal(shape) + 1j * d["rng"].standard_normal(shape)
+    Y = pb.forward(pb.image_to_plane(x)).cpu().numpy()
+    pl = pb.plan
+    n2 = shape[2]
+    for s in range(pb.nshots):
+        dense = O.encode_forward(x, d["mps_u"], d["A_u"][s], d["t"][s], d["okg"], d["orkg"])
+        hyb = np.fft.ifft(dense, axis=-1, norm="ortho")
+        o, c = pl.off_h[s], pl.cnt_h[s]
+        ref = hyb[:, pl.k0_h[o:o + c], pl.k1_h[o:o + c], :]
+        got = Y[o * n2 * nc:(o + c) * n2 * nc].reshape(n2, nc, c)
+        assert _rel(np.transpose(got, (1, 2, 0)), ref) < TOL, s
+
+
+def test_normal_operator_and_rhs(case):
+    d = case
+    pb, shape = d["pb"], d["shape"]
+    x = d["rng"].standard_normal(shape) + 1j * d["rng"].standard_normal(shape)
+    out = pb.zeros_image()
+    pb.normal(pb.image_to_plane(x), out)
+    ref = O.normal_op(x, d["mps_u"], d["A_u"], d["t"], d["okg"], d["orkg"], d["M"])
+    assert _rel(pb.image_from_plane(out), ref) < TOL
+    refb = O.cg_rhs(d["ksp"], d["mps_u"], d["A_u"], d["t"], d["okg"], d["orkg"], d["M"])
+    assert _rel(pb.image_from_plane(pb.rhs(cache=False)), refb) < TOL
+    assert _rel(pb.image_from_plane(pb.rhs(cache=True)), refb) < TOL
