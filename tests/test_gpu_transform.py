@@ -23,7 +23,10 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-SHAPES = [(16, 12, 8), (32, 24, 48), (64, 64, 48), (24, 96, 16)]
+SHAPES = [(16, 12, 8), (32, 24, 48), (64, 64, 48), (24, 96, 16),
+          # the long lines: 128, 192, 256, 384, 512 along the contiguous axis (1) and the
+          # strided axes (0: stride N1, 2: stride N0*N1)
+          (16, 384, 8), (512, 12, 8), (8, 16, 384), (128, 8, 192), (256, 16, 8)]
 PARAMS = [
     np.array([0.7, -1.3, 0.4, 0.11, -0.07, 0.05]),
     np.array([-2.5, 1.75, 3.0, np.radians(5), np.radians(-4), np.radians(3)]),
@@ -45,7 +48,7 @@ def test_apply_adjoint_match_oracle(me, shape, pi):
     assert _rel(Ti.apply(xt), O.rigid_apply(x.astype(np.complex128), p, okg, orkg, inverse=True)) < TOL
 
 
-@pytest.mark.parametrize("shape", [(16, 12, 8), (32, 24, 48)])
+@pytest.mark.parametrize("shape", [(16, 12, 8), (32, 24, 48), (128, 8, 192), (8, 512, 16)])
 @pytest.mark.parametrize("p_idx", range(6))
 def test_derivative_matches_oracle(me, shape, p_idx):
     p = PARAMS[1]
