@@ -80,8 +80,11 @@ __device__ __forceinline__ void phase_acc(const PhaseSpec& P, const double* cf, 
   const double c = P.ic >= 0 ? (double)P.sc * cf[P.ic] : 0.0;
   const double d = P.id >= 0 ? (double)P.sd * cf[P.id] : 0.0;
   const double beta = c * rb + d;  // angle = k_a * beta
-  const float2 cur = c_scale(cis2pi(frac_half(beta * (double)t / (double)N)), scale);
-  const float2 step = cis2pi(frac_half(beta * (double)R1 / (double)N));
+  // compile-time reciprocals instead of FP64 divisions (R1/N = 1/R2)
+  constexpr double kInvN = 1.0 / (double)N;
+  constexpr double kInvR2 = 1.0 / (double)R2;
+  const float2 cur = c_scale(cis2pi(frac_half(beta * ((double)t * kInvN))), scale);
+  const float2 step = cis2pi(frac_half(beta * kInvR2));
   const float2 wrap = cis2pi(frac_half(-beta));
   if (P.deriv) {  // uniform: one of two unrolled loops
     const double g = P.ig >= 0 ? (double)P.sg * cf[P.ig] : 0.0;
