@@ -4,7 +4,7 @@ Python mirror of the reference's hot-path API, over libmeamd.so (C ABI in
 include/motion_estimation_amd.h):
 
 * transform: RigidTransformCudaOptimzied, RigidTransformDerivativeCuda
-* utils: compute_transform_grids_voxel
+* utils: compute_transform_grids_voxel, generate_motion_parameters
 * image_estimation: estimate_image_cg (CG-SENSE)
 * transform_estimation: estimate_transform (per-shot Levenberg-Marquardt)
 * joint_recon: JointRecon
@@ -15,13 +15,15 @@ include/motion_estimation_amd.h):
   NIfTI-1 and npz outputs; metrics: error volume, gradient entropy
 * autograd: DataConsistencyLoss (gradients w.r.t. image and motion)
 * batched: BatchedJointRecon (many volumes, HIP graph replay, line-pass tuning)
-* simulation: DISORDER sampling, corrupted k-space, synthetic inputs
+* simulation: DISORDER sampling, corrupted k-space, the reference's seeded
+  motion / noise / sampling generators, run_experiment (the synthetic
+  experiment), synthetic benchmark inputs
 
 The package directory is named ``motion-estimation_amd``; load it with
 ``__graft_entry__.load_package()`` (registers it as ``motion_estimation_amd``).
 """
 from . import _lib  # noqa: F401
-from .utils import compute_transform_grids_voxel  # noqa: F401
+from .utils import compute_transform_grids_voxel, generate_motion_parameters  # noqa: F401
 from .transform import RigidTransformCudaOptimzied, RigidTransformDerivativeCuda  # noqa: F401
 from . import encoding, simulation  # noqa: F401
 from .image_estimation import estimate_image_cg  # noqa: F401

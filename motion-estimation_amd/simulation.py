@@ -1,7 +1,10 @@
 """Synthetic inputs: DISORDER sampling (simulation.py:192-245, 383-410), the
 motion-corrupted forward model (simulation.py:65-85), and the seeded
 phantom / coil / motion / noise generators the benchmark configurations
-are defined with.
+are defined with.  Also the reference's own NumPy generators
+(``generate_motion_parameters_new``, ``generate_complex_gaussian_noise``,
+``sample_disorder``; same draws, same outputs) and its synthetic experiment
+(simulation.py:250-493) as :func:`run_experiment`.
 
 DISORDER is computed by the library's host C code (bit-exact with the
 reference's stdlib ``random``); the corrupted k-space is synthesised on the GPU
@@ -234,6 +237,208 @@ def add_noise(ksp, snr_db, sampled, seed=3):
     ksp = ksp + noise
     noise = torch.randn(ksp.shape, generator=gen, device=ksp.device, dtype=torch.float32) * sigma
     return (ksp + 1j * noise).to(torch.complex64)
+
+
+# ------------------------------------------------ the reference's generators
+def generate_complex_gaussian_noise(shape, std=1.0, seed=42):
+    """simulation.py:99-102 with xp = numpy: std * N1 + 1j * std * N2 from
+    one NumPy generator (float32 draws, N1 first), complex64.  ``std`` may be
+    complex, as the experiment passes max(ksp) * 1e-4 * no.  The reference's
+    CuPy call (xp = cupy) draws from CuPy's generator instead, whose stream
+    is not reproduced here."""
+    rng = np.random.default_rng(seed)
+    n = std * rng.standard_normal(shape, dtype=np.float32) + 1j * std * rng.standard_normal(shape, dtype=np.float32)
+    return n.astype(np.complex64)
+
+
+def generate_motion_parameters_new(num_states, low_freq_var=0.1, high_freq_var=5.0, spike_prob=0.02, seed=123456789,
+                                   rot_limit_deg=20.0):
+    """simulation.py:104-174: per parameter a linear drift from 0 to a
+    Gaussian end point, a smooth AR(1) wobble (rho = 0.98, stationary
+    variance 0.3 / 0.2 x low_freq_var for translations / rotations) and one
+    or two Gaussian bumps (count round(spike_prob * T) clamped to [1, 2],
+    centres U[0.15, 0.85], widths U[0.06, 0.14] x T, amplitudes scaled from
+    high_freq_var); rotations (degrees) clipped to +-rot_limit_deg, then
+    radians.  Columns (trans x3 in voxels, rot x3 in radians).  The draws come
+    from one NumPy generator in the reference's order (drift ends, wobble
+    innovations, bump centres, widths, amplitudes), so the output is identical
+    for a given seed."""
+    rng = np.random.default_rng(seed)
+    T = int(num_states)
+    tt = np.linspace(0.0, 1.0, T)
+    end_trans = rng.normal(0, np.sqrt(low_freq_var), size=3)
+    end_rot = rng.normal(0, 0.6 * np.sqrt(low_freq_var), size=3)
+    rho = 0.98
+
+    def wobble(target_var):
+        sd = np.sqrt(max(1e-12, target_var) * (1 - rho ** 2))
+        w = np.zeros((T, 3))
+        for i in range(1, T):  # one draw of 3 per step, in step order
+            w[i] = rho * w[i - 1] + rng.normal(0, sd, size=3)
+        return w
+
+    wob_trans = wobble(0.3 * low_freq_var)
+    wob_rot = wobble(0.2 * low_freq_var)
+    nb = int(np.clip(max(1, int(round(spike_prob * T))), 1, 2))
+    centre = rng.uniform(0.15, 0.85, size=nb)
+    width = rng.uniform(0.06, 0.14, size=nb) * T
+    amp_trans = rng.normal(0, 0.15 * np.sqrt(high_freq_var), size=(nb, 3))
+    amp_rot = rng.normal(0, 0.08 * np.sqrt(high_freq_var), size=(nb, 3))
+    bump_trans = np.zeros((T, 3))
+    bump_rot = np.zeros((T, 3))
+    pos = np.arange(T)
+    for k in range(nb):
+        g = np.exp(-0.5 * ((pos - centre[k] * T) / (width[k] + 1e-9)) ** 2)[:, None]
+        bump_trans += g * amp_trans[k][None, :]
+        bump_rot += g * amp_rot[k][None, :]
+    trans = tt[:, None] * end_trans[None, :] + wob_trans + bump_trans
+    rot = tt[:, None] * end_rot[None, :] + wob_rot + bump_rot
+    lim = float(rot_limit_deg)
+    rot = np.deg2rad(np.clip(rot, -lim, lim))
+    return np.hstack([trans, rot])
+
+
+def sample_disorder(K, U, R=1, seed=42):
+    """simulation.py:24-63: DISORDER by NumPy permutations.  Sample k = (ky,
+    kx) of the K grid belongs to tile (ky // U0) * (K1 // U1) + kx // U1; for
+    each of the (K0 // U0) * (K1 // U1) tiles in turn its samples are permuted
+    by the generator, every R-th kept and numbered 0 .. prod(U)//R - 1 as
+    shot ids.  Returns (shot_ids (-1 where unsampled), temporal order of the
+    flat indices shot by shot, y_order, x_order).  A tile whose kept count is
+    not prod(U)//R raises ValueError, as the reference's assignment does."""
+    if not float(R).is_integer() or R < 1:
+        raise TypeError("R must be a positive integer (the reference slices with it)")
+    R = int(R)
+    K = (int(K[0]), int(K[1]))
+    ky, kx = np.meshgrid(np.arange(K[0]), np.arange(K[1]), indexing="ij")
+    ky, kx = ky.ravel(), kx.ravel()
+    rng = np.random.default_rng(seed)
+    ntx = K[1] // U[1]
+    ntiles = (K[0] // U[0]) * ntx
+    tile = (ky // U[0]).astype(int) * ntx + (kx // U[1]).astype(int)
+    shot_ids = -np.ones(K[0] * K[1], dtype=int)
+    nshots = int(np.prod(U)) // R
+    for t in range(ntiles):
+        ids = rng.permutation(np.flatnonzero(tile == t))[::R]
+        shot_ids[ids] = np.arange(nshots, dtype=int)
+    temporal = np.concatenate([np.flatnonzero(shot_ids == s) for s in range(nshots)])
+    y_order, x_order = np.unravel_index(temporal, K)
+    return shot_ids, temporal, y_order.astype(int), x_order.astype(int)
+
+
+def sequential_masks(K, nshots, accel):
+    """simulation.py:412-433: line-wise order, partitions 0, accel, 2 accel,
+    ... each with all K1 lines; shot s takes samples [s*n, (s+1)*n) with
+    n = len(order) // nshots (the remainder is dropped).  Returns (nshots, K0,
+    K1, 1) bool.  K0 must be a multiple of accel (the reference's two index
+    arrays differ in length otherwise; ValueError here)."""
+    K0, K1 = int(K[0]), int(K[1])
+    y = np.repeat(np.arange(0, K0, accel), K1)
+    x = np.tile(np.arange(K1), K0 // accel)
+    if y.shape != x.shape:
+        raise ValueError(f"K0 = {K0} is not a multiple of accel = {accel}")
+    A = np.zeros((nshots, K0, K1), dtype=bool)
+    n = y.shape[0] // nshots
+    for s in range(nshots):
+        A[s, y[s * n:(s + 1) * n], x[s * n:(s + 1) * n]] = True
+    return A[..., None]
+
+
+def _complex_max(x):
+    """NumPy's max of a complex array (lexicographic: largest real part, then
+    largest imaginary part among those), as a complex64 scalar; x a complex
+    torch tensor (evaluated where it lives)."""
+    re = x.real.max()
+    im = x.imag[x.real == re].max()
+    return np.complex64(complex(float(re), float(im)))
+
+
+def run_experiment(gt, mps, tile_size, accel=2, order="disorder", iters=200, noise=0, mask=None, low_freq_var=1,
+                   continuous=False, out_dir=None, sense_iter=100, pyramid_kwargs=None):
+    """The synthetic experiment of simulation.py:250-493 (without its CLI and
+    PNG previews).
+
+    gt (N0, N1, N2) and mps (C, N0, N1, N2) centred (NumPy, torch or
+    CuPy-like).  Steps, as the reference:
+
+    1. nshots = prod(tile_size) // accel recon shots; corruption states
+       N1 // accel with ``continuous``, else nshots;
+    2. masks: ``mask`` (K0, K1) for both, else DISORDER (order seed 123456)
+       or line-wise ``"sequential"``, binned per shot;
+    3. motion: generate_motion_parameters_new(states, low_freq_var,
+       high_freq_var=20);
+    4. corrupted k-space; with ``noise`` > 0 plus complex Gaussian noise of
+       std = max(ksp) * 1e-4 * noise (max in NumPy's complex order; the
+       noise from NumPy's generator, seed 42);
+    5. the uncorrected SENSE image (preconditioner 1 / (sum |S|^2 + 1e-6),
+       tol 1e-12, ``sense_iter`` iterations, SigPy's default 100);
+    6. pyramid_reconstruction(ksp, mps, A, nshots, n_joint_iters=iters) plus
+       ``pyramid_kwargs`` (the reference passes none, so its pyramid
+       downsamples the DISORDER way for sequential order too);
+    7. err_volume = |corrected - gt|, err_img its 2-norm.
+
+    With ``out_dir``: corrected.nii.gz / uncorrected.nii.gz (magnitudes,
+    float64), results.npz (simulation.py:488-493) and the pyramid's loss
+    histories.  Returns a dict with corrected, uncorrected, t_estimates,
+    err_volume, err_img, the masks, the corruption motion and the k-space."""
+    from .io import save_nifti
+    from .pyramid import pyramid_reconstruction
+    from .sense import sense_recon
+
+    dev = _lib.require_gpu()
+    gt_d = as_c64(gt, dev)
+    mps_d = as_c64(mps, dev)
+    if gt_d.dim() != 3:
+        raise AssertionError("gt must be 3-D")
+    if mps_d.dim() != 4 or tuple(mps_d.shape[1:]) != tuple(gt_d.shape):
+        raise AssertionError("mps must be (coils, *gt.shape)")
+    tile = tuple(int(v) for v in tile_size)
+    if len(tile) != 2:
+        raise AssertionError("tile_size must hold two integers (partitions, lines)")
+    shape = tuple(gt_d.shape)
+    if shape[0] % tile[0] or shape[1] % tile[1]:
+        raise AssertionError("gt's first two axes must be multiples of the tile")
+    nshots = tile[0] * tile[1] // accel
+    nstates = shape[1] // accel if continuous else nshots
+    K = shape[:2]
+    if mask is not None:
+        A = np.asarray(mask, dtype=bool)[..., None]
+        A_corr = A
+    elif order == "disorder":
+        cfg = DisorderConfig(max_line_number=shape[1] - 1, max_partition_number=shape[0] - 1,
+                             pat_lines_to_measure=shape[1] // accel, block_lin=tile[1], block_par=tile[0],
+                             seed=123456)
+        ordr, _ = make_disorder_order(cfg)
+        A = disorder_masks(ordr, nshots, K)
+        A_corr = disorder_masks(ordr, nstates, K)
+    elif order == "sequential":
+        A = sequential_masks(K, nshots, accel)
+        A_corr = sequential_masks(K, nstates, accel)
+    else:
+        raise ValueError("order must be 'disorder' or 'sequential'")
+    t_corr = generate_motion_parameters_new(nstates, low_freq_var=low_freq_var, high_freq_var=20.0)[:nstates]
+    ksp = generate_corrupted_kspace(gt_d, mps_d, A_corr, t_corr)
+    if noise > 0:
+        std = _complex_max(ksp) * np.float32(0.0001) * np.float32(noise)
+        ksp = ksp + torch.as_tensor(generate_complex_gaussian_noise(tuple(ksp.shape), std=std), device=dev)
+    ssq = (mps_d.abs() ** 2).sum(0)
+    P = (1.0 / (ssq + 1e-6)).cpu().numpy()
+    uncorrected = sense_recon(ksp, mps_d, P=P, tol=1e-12, max_iter=sense_iter).cpu().numpy()
+    corrected, t_est = pyramid_reconstruction(ksp, mps_d, A, nshots, n_joint_iters=iters, save_path=out_dir,
+                                              **(pyramid_kwargs or {}))
+    gt_h = gt_d.cpu().numpy()
+    err_volume = np.real(np.abs(corrected - gt_h))
+    err_img = float(np.linalg.norm(err_volume))
+    res = dict(corrected=corrected, uncorrected=uncorrected, t_estimates=t_est, err_volume=err_volume,
+               err_img=err_img, A=A, A_corr=A_corr, t_corr=t_corr, ksp=ksp)
+    if out_dir is not None:
+        import os
+        os.makedirs(out_dir, exist_ok=True)
+        save_nifti(os.path.join(str(out_dir), "corrected.nii.gz"), np.abs(corrected).astype(np.float64))
+        save_nifti(os.path.join(str(out_dir), "uncorrected.nii.gz"), np.abs(uncorrected).astype(np.float64))
+        np.savez(os.path.join(str(out_dir), "results"), corrected=np.abs(corrected), uncorrected=np.abs(uncorrected),
+                 t_estimates=t_est, err_volume=err_volume, err_img=err_img)
+    return res
 
 
 CONFIGS = {

@@ -1,4 +1,5 @@
-"""Grid construction (utils.py:107-137) and tensor plumbing helpers."""
+"""Grid construction (utils.py:107-137), the random-walk motion generator
+(utils.py:5-36) and tensor plumbing helpers."""
 from __future__ import annotations
 
 import numpy as np
@@ -50,6 +51,23 @@ def compute_transform_grids_voxel(shape, voxel_size, downsample_res=None, xp=np)
         g.r = rvec
         g.shape = (int(dsx), int(dsy), int(dsz))
     return kgrid, rkgrid
+
+
+def generate_motion_parameters(num_states, low_freq_var=0.1, high_freq_var=5.0, spike_prob=0.02, seed=123456789):
+    """utils.py:5-36: a Gaussian random walk per parameter (variance
+    ``low_freq_var`` per step) plus sparse spikes (each entry with probability
+    ``spike_prob``, Gaussian of variance ``high_freq_var``); the last three
+    columns are then read as degrees and converted to radians (no clipping).
+    Same NumPy generator and draw order as the reference, so the output is
+    identical for a given seed.  Returns (num_states, 6) float64."""
+    rng = np.random.default_rng(seed)
+    shape = (num_states, 6)
+    walk = np.cumsum(rng.normal(0, np.sqrt(low_freq_var), size=shape), axis=0)
+    spike_on = rng.choice([0, 1], size=shape, p=[1 - spike_prob, spike_prob])
+    spike_amp = rng.normal(0, np.sqrt(high_freq_var), size=shape)
+    out = walk + spike_on * spike_amp
+    out[:, 3:] *= np.pi / 180
+    return out
 
 
 def _device():
