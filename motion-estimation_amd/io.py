@@ -9,6 +9,9 @@ extract_from_dat.py:6-60, simulation.py:476-493).
 * ``prepare_real_data``: the calibration chain of recon.py:130-150 on the
   GPU (noise whitening, ESPIRiT from the reference scan cropped to 24^3,
   coil compression).
+* ``reconstruct_real_data``: the rest of recon.py:129-193 from those arrays
+  (SENSE image, pyramid motion correction over the counter masks, the pre/
+  and recon/ outputs).
 * ``save_nifti`` / ``load_nifti``: NIfTI-1 single files (.nii or .nii.gz)
   without nibabel, as nib.save(Nifti1Image(data, np.eye(4))) is used at
   simulation.py:480-481.
@@ -71,12 +74,13 @@ def _centred_resize(x, shape):
 
 
 def prepare_real_data(ksp, ref, noise, lines=None, partitions=None, n_shots=None, calib_size=24,
-                      kernel_size=(5, 5, 5), eig_thresh=0.02, mask_thresh=0.99, perc=0.95):
+                      kernel_size=(5, 5, 5), eig_thresh=0.02, mask_thresh=0.99, perc=0.95, keep_intermediate=False):
     """recon.py:130-165 on the GPU: whiten ksp with the noise covariance, ESPIRiT
     maps from the reference scan cropped/padded to calib_size^3 (not whitened,
     as in the reference), coil compression of maps and whitened k-space, and
     the shot masks from the counters when given.  ksp/ref: (coils, par, lin,
-    col); noise: coils first.  Returns dict(ksp, mps, shot_mask, W)."""
+    col); noise: coils first.  Returns dict(ksp, mps, shot_mask, W), with
+    ``keep_intermediate`` also white_ksp and the uncompressed maps mps_full."""
     dev = _lib.require_gpu()
     k = as_c64(ksp, dev)
     C = k.shape[0]
@@ -88,9 +92,54 @@ def prepare_real_data(ksp, ref, noise, lines=None, partitions=None, n_shots=None
     mps = mps.permute(3, 0, 1, 2).contiguous()
     cc_mps, cc_ksp = coil_compression(mps, white, perc=perc)
     out = dict(ksp=cc_ksp, mps=cc_mps, W=W, shot_mask=None)
+    if keep_intermediate:
+        out.update(white_ksp=white, mps_full=mps)
     if lines is not None:
         out["shot_mask"] = shot_masks_from_counters(lines, partitions, n_shots, white.shape[1], white.shape[2])
     return out
+
+
+def reconstruct_real_data(ksp, ref, noise, lines, partitions, n_shots, out_dir=None, sampling="disorder", iters=1000,
+                          calib_size=24, sense_iter=100, pyramid_kwargs=None):
+    """recon.py:129-193 from the arrays the TWIX extraction yields (ksp and
+    ref as (coils, par, lin, col), noise coils first, the cLin / cPar
+    counters of the imaging readouts in acquisition order):
+
+    1. :func:`prepare_real_data`: whitening, ESPIRiT maps (kernel 5^3,
+       threshold 0.02, mask 0.99) from the reference scan resized to
+       calib_size^3, coil compression to 95% and the shot masks;
+    2. the SENSE image of the compressed data, SigPy's defaults (no
+       preconditioner, ``sense_iter`` = 100 iterations, rss(y) > 0 weights);
+    3. pyramid_reconstruction(ksp, mps, masks, n_shots, n_joint_iters=iters,
+       sampling=sampling) plus ``pyramid_kwargs`` (the reference passes none).
+
+    With ``out_dir``: pre/ holds ksp, ref, noise, white_ksp, mps, cc_mps,
+    cc_ksp (.npy) and recon/ holds sense.npy, joint_recon.npy,
+    estimated_transforms.npy, joint_recon.nii.gz (magnitude, float64) and
+    the pyramid's loss histories, as the reference writes them.  Returns a
+    dict with the prepared arrays, sense, recon and t_est."""
+    from .pyramid import pyramid_reconstruction
+    from .sense import sense_recon
+
+    prep = prepare_real_data(ksp, ref, noise, lines=lines, partitions=partitions, n_shots=n_shots,
+                             calib_size=calib_size, keep_intermediate=True)
+    sense = sense_recon(prep["ksp"], prep["mps"], max_iter=sense_iter).cpu().numpy()
+    rec_dir = None if out_dir is None else os.path.join(str(out_dir), "recon")
+    recon, t_est = pyramid_reconstruction(prep["ksp"], prep["mps"], prep["shot_mask"], n_shots, n_joint_iters=iters,
+                                          save_path=rec_dir, sampling=sampling, **(pyramid_kwargs or {}))
+    if out_dir is not None:
+        pre = os.path.join(str(out_dir), "pre")
+        os.makedirs(pre, exist_ok=True)
+        os.makedirs(rec_dir, exist_ok=True)
+        host = lambda x: np.asarray(_to_numpy(x))  # noqa: E731
+        for name, arr in (("ksp", ksp), ("ref", ref), ("noise", noise), ("white_ksp", prep["white_ksp"]),
+                          ("mps", prep["mps_full"]), ("cc_mps", prep["mps"]), ("cc_ksp", prep["ksp"])):
+            np.save(os.path.join(pre, f"{name}.npy"), host(arr))
+        np.save(os.path.join(rec_dir, "sense.npy"), sense)
+        np.save(os.path.join(rec_dir, "joint_recon.npy"), recon)
+        np.save(os.path.join(rec_dir, "estimated_transforms.npy"), t_est)
+        save_nifti(os.path.join(rec_dir, "joint_recon.nii.gz"), np.abs(recon).astype(np.float64))
+    return dict(prep, sense=sense, recon=recon, t_est=t_est)
 
 
 # ------------------------------------------------------------------ NIfTI-1

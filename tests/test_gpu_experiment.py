@@ -4,12 +4,15 @@ motion generator, corrupted k-space with noise of std max(ksp) 1e-4, the SENSE
 baseline and the pyramid.  Small size (32 x 32 x 8, 4 coils), one joint
 iteration per level, 2 levels and 20 final CG iterations (the pyramid test's
 regime).  Relative L2 1e-4 on k-space and images, 1e-3 (degrees, voxels) on
-motion; the written NIfTI and npz files are read back."""
+motion; the written NIfTI and npz files are read back.  The real-data driver
+(recon.py:129-193) likewise, from synthetic scanner arrays."""
 import os
 
 import numpy as np
 import pytest
+import torch
 
+from espirit_data import espirit_data
 from oracle import motion_oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -86,3 +89,39 @@ def test_run_experiment_argument_checks(me):
         sim.run_experiment(gt, mps[:, :16], tile_size=(4, 4))
     with pytest.raises(ValueError):
         sim.run_experiment(gt, mps, tile_size=(4, 4), order="radial")
+
+
+@pytest.mark.parametrize("sampling", ["disorder", "sequential"])
+def test_reconstruct_real_data(me, tmp_path, sampling):
+    """Its calibration equals prepare_real_data (bitwise), and the SENSE image
+    and the pyramid match the oracle run on the same compressed data and
+    counter masks."""
+    imsize, nc, nshots = (24, 16, 16), 6, 4
+    calib, _, _ = espirit_data(imsize, nc=nc, cal=imsize, seed=9)
+    ksp = np.transpose(calib, (3, 0, 1, 2)).astype(np.complex64)
+    rng = np.random.default_rng(6)
+    noise = (0.01 * (rng.standard_normal((nc, 3000)) + 1j * rng.standard_normal((nc, 3000)))).astype(np.complex64)
+    order = rng.permutation(imsize[0] * imsize[1])
+    parts, lines = order // imsize[1], order % imsize[1]
+    pk = dict(n_spatial_levels=2, final_cg_iter=20)
+    res = me.io.reconstruct_real_data(ksp, ksp, noise, lines, parts, nshots, out_dir=str(tmp_path), sampling=sampling,
+                                      iters=1, calib_size=12, sense_iter=20, pyramid_kwargs=pk)
+    prep = me.io.prepare_real_data(ksp, ksp, noise, lines=lines, partitions=parts, n_shots=nshots, calib_size=12)
+    assert torch.equal(res["ksp"], prep["ksp"]) and torch.equal(res["mps"], prep["mps"])
+    assert np.array_equal(res["shot_mask"], O.shot_masks_from_counters(lines, parts, nshots, imsize[0], imsize[1]))
+    y = res["ksp"].cpu().numpy().astype(np.complex128)
+    s = res["mps"].cpu().numpy().astype(np.complex128)
+    sense = O.sense_recon(y, s, max_iter=20)
+    assert _rel(res["sense"], sense) < TOL
+    x_ref, t_ref = O.pyramid_reconstruction(y, s, res["shot_mask"], nshots, n_joint_iters=1, sampling=sampling, **pk)
+    assert _rel(res["recon"], x_ref) < TOL
+    d = np.abs(res["t_est"] - t_ref)
+    assert max(d[:, :3].max(), np.degrees(d[:, 3:]).max()) < 1e-3
+    pre, rec = tmp_path / "pre", tmp_path / "recon"
+    for name in ("ksp", "ref", "noise", "white_ksp", "mps", "cc_mps", "cc_ksp"):
+        assert (pre / f"{name}.npy").exists(), name
+    assert np.array_equal(np.load(pre / "cc_ksp.npy"), res["ksp"].cpu().numpy())
+    assert np.array_equal(np.load(rec / "joint_recon.npy"), res["recon"])
+    assert np.array_equal(np.load(rec / "estimated_transforms.npy"), res["t_est"])
+    data, _ = me.io.load_nifti(str(rec / "joint_recon.nii.gz"))
+    assert np.array_equal(data, np.abs(res["recon"]).astype(np.float64))
