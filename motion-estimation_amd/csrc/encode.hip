@@ -150,6 +150,10 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
       uv[m1] = up[R2 * m1 + t];
     }
   }
+  // the thread's row-FFT twiddles exp(-2 pi i t k1 / NR), fixed over the rows
+  float2 tw1[R1];
+#pragma unroll
+  for (int k1 = 1; k1 < R1; ++k1) tw1[k1] = twr[(t * k1) % NR];
   // twiddle index (k0 * x0) mod n0 of each sample, advanced row by row
   int tix[SPT];
 #pragma unroll
@@ -169,7 +173,7 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
       }
       RDFT<R1, -1>::run(a);
 #pragma unroll
-      for (int k1 = 1; k1 < R1; ++k1) a[k1] = c_mul(a[k1], twr[(t * k1) % NR]);
+      for (int k1 = 1; k1 < R1; ++k1) a[k1] = c_mul(a[k1], tw1[k1]);
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) xch[lidx(k1, t)] = a[k1];
     }
