@@ -84,7 +84,11 @@ template <int NR, int SPT>
 __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
   constexpr int R1 = RowT<NR>::R1, R2 = RowT<NR>::R2, T = RowT<NR>::T, NT = RowT<NR>::NT;
   constexpr int G = kCoilsPerWG;
-  __shared__ float2 xch[G * R1 * (R2 + 1)];
+  // exchange rows padded to XS (even, so each row starts 16-byte aligned and
+  // the step-2 reads go in pairs)
+  constexpr int XS = R2 + 2;
+  static_assert(R2 % 2 == 0, "paired exchange reads need an even R2");
+  __shared__ __align__(16) float2 xch[G * R1 * XS];
   // row spectrum, coil-fastest per column k1 (stride ZS = G + 2 keeps 16-byte
   // alignment for paired coil reads and spreads the step-2 writes over banks)
   constexpr int ZS = G + 2;
@@ -135,7 +139,7 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
   const long long vol = plane * n2;
   const float2* up = A.u + (long long)b * A.u_bs + (long long)x2 * plane;
   const float2* sp = A.maps + (long long)(cvalid ? c : 0) * vol + (long long)x2 * plane;
-  auto lidx = [&](int k1, int m2) { return g * (R1 * (R2 + 1)) + k1 * (R2 + 1) + m2; };
+  auto lidx = [&](int k1, int m2) { return g * (R1 * XS) + k1 * XS + m2; };
   __syncthreads();
 
   // rows are streamed with the next row's map/image values prefetched into
@@ -181,7 +185,11 @@ __global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
     if (t < R1) {
       float2 X[R2];
 #pragma unroll
-      for (int m2 = 0; m2 < R2; ++m2) X[m2] = xch[lidx(t, m2)];
+      for (int m2 = 0; m2 < R2; m2 += 2) {
+        const float4 q = *reinterpret_cast<const float4*>(&xch[lidx(t, m2)]);
+        X[m2] = make_float2(q.x, q.y);
+        X[m2 + 1] = make_float2(q.z, q.w);
+      }
       RDFT<R2, -1>::run(X);
 #pragma unroll
       for (int k2 = 0; k2 < R2; ++k2) zrow[(t + R1 * k2) * ZS + g] = X[k2];
