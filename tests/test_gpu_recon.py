@@ -269,3 +269,34 @@ def test_cg_tracking_and_initial_norms(me, prob_data):
     np.testing.assert_allclose(norms0.cpu().numpy(), obj0, rtol=1e-6)
     fresh = pb.forward(x).clone()
     assert _rel(track, fresh.cpu().numpy()) < TOL
+
+
+def test_empty_shot_matches_oracle(me, prob_data):
+    """A shot with no samples (a user mask can hold one): it contributes
+    nothing to the normal operator and right-hand side, and its LM step is
+    rejected (e1 = e0 = 0), so its damping grows by 1.5 as in the reference."""
+    d = prob_data
+    A_u = d["A_u"].copy()
+    A_u[3] = False
+    pb = me.encoding.SenseProblem(d["ksp_u"], d["mps_u"], A_u, d["kg"], d["rkg"])
+    assert int(pb.plan.cnt_h[3]) == 0
+    pb.theta.copy_(torch.as_tensor(d["t_true"]))
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal(SHAPE) + 1j * rng.standard_normal(SHAPE)
+    out = pb.zeros_image()
+    pb.normal(pb.image_to_plane(x), out)
+    ref = O.normal_op(x, d["mps_u"], A_u, d["t_true"], d["okg"], d["orkg"], d["M"])
+    assert _rel(pb.image_from_plane(out), ref) < TOL
+    refb = O.cg_rhs(d["ksp_u"], d["mps_u"], A_u, d["t_true"], d["okg"], d["orkg"], d["M"])
+    assert _rel(pb.image_from_plane(pb.rhs(cache=False)), refb) < TOL
+    xi = O.estimate_image_cg(d["ksp_u"], d["mps_u"], A_u, d["t_true"], d["okg"], d["orkg"], d["P"], d["M"], max_iter=5)
+    t0 = d["t_true"] + np.random.default_rng(7).normal(0, 0.05, d["t_true"].shape)
+    ns = t0.shape[0]
+    t_ref, damp_ref, fl_ref = t0.copy(), np.ones(ns), np.zeros(ns, dtype=bool)
+    x_ref = O.estimate_transform(d["ksp_u"], d["mps_u"], A_u, xi, d["okg"], d["orkg"], damp_ref, fl_ref, t_ref)
+    t_got, damp_got, fl_got = t0.copy(), np.ones(ns), np.zeros(ns, dtype=bool)
+    x_got = me.estimate_transform(d["ksp_u"], d["mps_u"], A_u, xi, d["kg"], d["rkg"], damp_got, fl_got, None, None,
+                                  None, None, transforms=t_got)
+    assert damp_ref[3] == 1.5 and np.array_equal(damp_got, damp_ref)
+    assert _motion_err(t_got, t_ref) < MOTION_TOL
+    assert _rel(x_got, x_ref) < TOL
