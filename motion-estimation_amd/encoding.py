@@ -22,6 +22,13 @@ from .utils import as_c64, _to_numpy
 CG_DONE = 5
 
 
+# Working-set budget (bytes) that sets how many shots a batch holds: the
+# per-shot work volumes (transformed images, partial images) of one batch
+# stay within it.  8 GiB keeps all 64 shots of a 256 x 256 x 192 problem in
+# one batch.
+DEFAULT_MEM_BUDGET = 8 << 30
+
+
 def _i4(v):
     return (C.c_int * 4)(*[int(a) for a in v])
 
@@ -187,7 +194,7 @@ class SenseProblem:
         self.P = real_to_plane(P, self.shape, torch.float32, dev) if P is not None else Pp
         self.M = real_to_plane(M, self.shape, torch.uint8, dev) if M is not None else Mp
         # batching over shots
-        budget = mem_budget or int(8 * 2**30)
+        budget = mem_budget or DEFAULT_MEM_BUDGET
         ncg = int(_lib.size("me_encode_coil_groups", self.ncoils))
         self.batch = int(max(1, min(self.nshots, budget // max(1, vol * 8 * ncg))))
         self.theta = torch.zeros((self.nshots, 6), dtype=torch.float64, device=dev)
