@@ -85,6 +85,25 @@ def test_downsampled_grid_mixed_units(me):
     assert _rel(T2.apply(xt), O.rigid_apply(x.astype(np.complex128), p, okg, orkg)) < TOL
 
 
+@pytest.mark.parametrize("voxel", [(1.2, 0.8, 1.5), (0.5, 0.5, 2.0)])
+def test_anisotropic_voxels(me, voxel):
+    """Voxel sizes other than 1 scale r (utils.py:114-116): apply, adjoint and
+    one rotation and one translation derivative column."""
+    shape = (32, 24, 16)
+    x = _vol(shape, 5)
+    p = PARAMS[1]
+    kg, rkg = me.compute_transform_grids_voxel(shape, voxel)
+    okg, orkg = O.compute_transform_grids_voxel(shape, voxel)
+    xt = torch.as_tensor(x, device="cuda")
+    x64 = x.astype(np.complex128)
+    T = me.RigidTransformCudaOptimzied(p, kg, rkg)
+    assert _rel(T.apply(xt), O.rigid_apply(x64, p, okg, orkg)) < TOL
+    assert _rel(T.adjoint(xt), O.rigid_adjoint(x64, p, okg, orkg)) < TOL
+    dT = me.RigidTransformDerivativeCuda(shape, p, kg, rkg)
+    for p_idx in (1, 4):
+        assert _rel(dT.apply(xt, p_idx), O.rigid_deriv(x64, p, okg, orkg, p_idx)) < TOL
+
+
 def test_apply_out_argument_quirk(me):
     """transform.py:265: with out=..., apply transforms out's contents."""
     shape = (16, 12, 8)
