@@ -372,6 +372,13 @@ long long me_jacobi_workspace_size(int m);
 int me_jacobi_eig(double* G, double* V, int m, int max_sweeps, double tol, double* eigvals, double* work,
                   int* sweeps_done, hipStream_t stream);
 
+/* Largest m for which me_jacobi_eig keeps the rows of V in LDS and applies 32
+ * rounds of rotations per pass (default and maximum 8192); above it V is
+ * rotated in global memory inside every round.  Returns the previous value.
+ * Process-wide; no counterpart in the reference (a control for exercising
+ * the large-matrix path at small sizes). */
+int me_jacobi_set_lds_rows(int rows);
+
 /* P = V_r V_r^H for the r columns idx[0..r) (device ints) of V (m rows of
  * ldv interleaved FP64 complex): the projector onto the kept singular
  * subspace of espirit.py:118-137.  Xwork holds m x r complex64. */

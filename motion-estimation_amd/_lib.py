@@ -90,6 +90,7 @@ _SIZES = {
     "me_line_tuning_get": ([vp, vp, i32], C.c_int),
     "me_metric_partials_size": ([], C.c_int),
     "me_jacobi_workspace_size": ([i32], C.c_longlong),
+    "me_jacobi_set_lds_rows": ([i32], C.c_int),
     "me_espirit_maps_workspace_size": ([i32, vp, vp], C.c_longlong),
 }
 _OPTIONAL = {}

@@ -248,6 +248,9 @@ __global__ void __launch_bounds__(256) k_jac_round(double* __restrict__ G, doubl
 // rots + k * npairs * 4), so V makes one HBM round trip per nr rounds.
 constexpr int kRoundsPerVPass = 32;
 constexpr int kMaxLdsRow = 8192;  // complex FP64 entries of a row in LDS
+// rows up to which V is updated in LDS (kMaxLdsRow, lowered only to test
+// the in-round global update at small sizes)
+static int g_jac_lds_rows = kMaxLdsRow;
 __global__ void __launch_bounds__(256) k_jac_apply_v(double* __restrict__ V, int m, int mm, int r0, int nr,
                                                      const double* __restrict__ rots) {
   extern __shared__ double2 vrow[];
@@ -740,13 +743,19 @@ long long me_jacobi_workspace_size(int m) {
   return (long long)kRoundsPerVPass * ((m + 2) / 2) * 4 + 2 * kNormBlocks;
 }
 
+int me_jacobi_set_lds_rows(int rows) {
+  const int prev = g_jac_lds_rows;
+  g_jac_lds_rows = rows < 0 ? 0 : (rows > kMaxLdsRow ? kMaxLdsRow : rows);
+  return prev;
+}
+
 int me_jacobi_eig(double* G, double* V, int m, int max_sweeps, double tol, double* eigvals, double* work,
                   int* sweeps_done, hipStream_t stream) {
   ME_REQUIRE(G && V && eigvals && work && m >= 1, "jacobi_eig: bad argument");
   ME_REQUIRE(m <= 32768, "jacobi_eig: at most 32768 rows");
   const int mm = m + (m & 1);
   const int npairs = mm / 2;
-  const bool lds_v = m <= kMaxLdsRow;
+  const bool lds_v = m <= g_jac_lds_rows;
   double* rots = work;  // kRoundsPerVPass rounds of npairs x 4
   double* part = work + (long long)kRoundsPerVPass * npairs * 4;
   const size_t vshm = (size_t)m * sizeof(double2);

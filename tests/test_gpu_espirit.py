@@ -65,6 +65,26 @@ def _jacobi(me, A, sweeps=40, tol=1e-14):
     return lam.cpu().numpy(), torch.view_as_complex(V).cpu().numpy(), done.value
 
 
+@pytest.mark.parametrize("m", [5, 64, 127])
+def test_jacobi_global_v_updates(me, m):
+    """The path for matrices above 8192 rows (V rotated in global memory in
+    every round instead of 32 rounds at a time in LDS), forced at small m:
+    the same rotations in the same order, so the same eigenpairs."""
+    rng = np.random.default_rng(30 + m)
+    X = rng.standard_normal((m, m + 2)) + 1j * rng.standard_normal((m, m + 2))
+    A = X @ X.conj().T
+    lam0, V0, sw0 = _jacobi(me, A)
+    prev = me._lib.size("me_jacobi_set_lds_rows", 0)
+    try:
+        lam1, V1, sw1 = _jacobi(me, A)
+    finally:
+        me._lib.size("me_jacobi_set_lds_rows", prev)
+    assert prev == 8192 and sw1 == sw0
+    assert np.array_equal(lam1, lam0)
+    assert np.linalg.norm(V1 - V0) <= 1e-13 * np.sqrt(m)
+    assert np.linalg.norm(V1 @ np.diag(lam1) @ V1.conj().T - A) <= 1e-12 * np.linalg.norm(A)
+
+
 @pytest.mark.parametrize("m", [1, 2, 5, 64, 127])
 def test_jacobi_eig_matches_eigh(me, m):
     rng = np.random.default_rng(10 + m)
