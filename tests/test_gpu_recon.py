@@ -300,3 +300,17 @@ def test_empty_shot_matches_oracle(me, prob_data):
     assert damp_ref[3] == 1.5 and np.array_equal(damp_got, damp_ref)
     assert _motion_err(t_got, t_ref) < MOTION_TOL
     assert _rel(x_got, x_ref) < TOL
+
+
+def test_masks_along_the_readout_axis(me, prob_data):
+    """Quirk Q16: the readout is axis 2 and the masks are singleton there.  A
+    mask repeated along axis 2 is the same mask; one that varies along it
+    cannot be encoded in hybrid space and is refused."""
+    d = prob_data
+    rep = np.repeat(d["A_u"], SHAPE[2], axis=3)
+    a = me.encoding.ShotPlan(d["A_u"], torch.device("cuda:0"))
+    b = me.encoding.ShotPlan(rep, torch.device("cuda:0"))
+    assert np.array_equal(a.cnt_h, b.cnt_h) and np.array_equal(a.k0_h, b.k0_h) and np.array_equal(a.k1_h, b.k1_h)
+    rep[0, 0, 0, 1] = not rep[0, 0, 0, 0]
+    with pytest.raises(ValueError, match="readout axis"):
+        me.encoding.ShotPlan(rep, torch.device("cuda:0"))
