@@ -363,8 +363,10 @@ def run_experiment(gt, mps, tile_size, accel=2, order="disorder", iters=200, noi
 
     1. nshots = prod(tile_size) // accel recon shots; corruption states
        N1 // accel with ``continuous``, else nshots;
-    2. masks: ``mask`` (K0, K1) for both, else DISORDER (order seed 123456)
-       or line-wise ``"sequential"``, binned per shot;
+    2. masks: ``mask`` (shots, K0, K1), used for both the reconstruction and
+       the corruption (with the continuous option ignored, as in the
+       reference), else DISORDER (order seed 123456) or line-wise
+       ``"sequential"``, binned per shot;
     3. motion: generate_motion_parameters_new(states, low_freq_var,
        high_freq_var=20);
     4. corrupted k-space; with ``noise`` > 0 plus complex Gaussian noise of
