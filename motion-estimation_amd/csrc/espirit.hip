@@ -393,8 +393,8 @@ __global__ void k_espirit_az(const float2* __restrict__ Q, int NRx, int kx, int 
 // largest diagonal entry), until the
 // estimated distance to the limit, |dv| rho / (1 - rho) with rho the ratio
 // of successive update norms, is below 1e-10 (or iters steps); a voxel
-// whose trace already fails the mask (lambda_max <= trace for a PSD matrix)
-// skips the iteration.
+// whose trace or Frobenius norm already fails the mask (both bound
+// lambda_max from above for a PSD matrix) skips the iteration.
 constexpr int kVoxThreads = 256;
 // B[dx][c2][c] of image row (y, z) into dst (LDS or global)
 __device__ __forceinline__ void espirit_row_table(const float2* __restrict__ Az, int NRx, int kx, int ky, int Ny,
@@ -488,8 +488,18 @@ __global__ void __launch_bounds__(kVoxThreads) k_espirit_voxels(const float2* __
         arg = oa;
       }
     }
+    // ||Cs||_F >= lambda_max for the PSD Cs: below the threshold (less a
+    // margin for FP32 rounding) the voxel cannot be kept either
+    float fr = 0.f;
+    if (act) {
+#pragma unroll
+      for (int c2 = 0; c2 < G2; ++c2) fr = fmaf(row[c2].x, row[c2].x, fmaf(row[c2].y, row[c2].y, fr));
+    }
+#pragma unroll
+    for (int o = G2 / 2; o > 0; o >>= 1) fr += __shfl_xor(fr, o, G2);
+    const bool frob_out = (double)fr <= (double)thr2k * (double)thr2k * (1.0 - 1e-4);
     float2 outv = make_float2(0.f, 0.f);
-    if (tr > thr2k) {  // group-uniform
+    if (tr > thr2k && !frob_out) {  // group-uniform
       // FP64 iterate (FP32 matrix): the update norm falls to ~1e-15, so the
       // error estimate dif * rho / (1 - rho) stays meaningful for slow rates
       double vx = 0.0, vy = 0.0;
