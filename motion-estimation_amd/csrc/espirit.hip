@@ -392,7 +392,8 @@ __global__ void k_espirit_az(const float2* __restrict__ Q, int NRx, int kx, int 
 // same lane group when that voxel was kept (else from the column of the
 // largest diagonal entry), until the
 // estimated distance to the limit, |dv| rho / (1 - rho) with rho the ratio
-// of successive update norms, is below 1e-10 (or iters steps); a voxel
+// of successive update norms, is below 1e-8, under the float32 rounding of
+// the stored maps (or iters steps); a voxel
 // whose trace or Frobenius norm already fails the mask (both bound
 // lambda_max from above for a PSD matrix) skips the iteration.
 constexpr int kVoxThreads = 256;
@@ -461,12 +462,16 @@ __global__ void __launch_bounds__(kVoxThreads) k_espirit_voxels(const float2* __
 #pragma unroll
     for (int c2 = 0; c2 < G2; ++c2) row[c2] = make_float2(0.f, 0.f);
     if (act) {
+      // phases exp(i theta_x d), d = -(kx-1) .. kx-1: the first exactly, the
+      // rest by the step exp(i theta_x) (a few float32 ulps over 2 kx - 2 steps)
+      float2 ph = esp_phase(x, Nx, -(kx - 1));
+      const float2 step = esp_phase(x, Nx, 1);
       for (int j = 0; j < Dx; ++j) {
-        const float2 ph = esp_phase(x, Nx, j - (kx - 1));
         const float2* bj = Bs + (size_t)j * NRx * NRx + c;
 #pragma unroll
         for (int c2 = 0; c2 < G2; ++c2)
           if (c2 < NRx) row[c2] = c_fma(bj[c2 * NRx], ph, row[c2]);
+        ph = c_mul(ph, step);
       }
     }
     // diagonal (real) of this lane's row; trace and argmax over the group
@@ -562,7 +567,7 @@ __global__ void __launch_bounds__(kVoxThreads) k_espirit_voxels(const float2* __
         dif = sqrt(dif);
         const double rho = fmin(dif / fmax(prev, 1e-300), 0.999999);
         prev = dif;
-        if (dif * rho <= 1e-10 * (1.0 - rho) || dif <= 1e-14) break;  // group-uniform
+        if (dif * rho <= 1e-8 * (1.0 - rho) || dif <= 1e-14) break;  // group-uniform
       }
       // lam: Rayleigh quotient of the last iterate, lambda_max(Cs)
       const bool valid = lam > (double)thr2k;  // group-uniform
