@@ -25,6 +25,7 @@
 //     with lane c of a lane group holding row c, the mask
 //     sqrt(lambda_max(C)) * prod(N) / sqrt(prod(kernel)) > mask_thresh, and
 //     the phase of coil 0 removed (espirit.py:200-239).
+#include <atomic>
 #include <vector>
 
 #include "me_device.h"
@@ -250,7 +251,7 @@ constexpr int kRoundsPerVPass = 32;
 constexpr int kMaxLdsRow = 8192;  // complex FP64 entries of a row in LDS
 // rows up to which V is updated in LDS (kMaxLdsRow, lowered only to test
 // the in-round global update at small sizes)
-static int g_jac_lds_rows = kMaxLdsRow;
+static std::atomic<int> g_jac_lds_rows{kMaxLdsRow};
 __global__ void __launch_bounds__(256) k_jac_apply_v(double* __restrict__ V, int m, int mm, int r0, int nr,
                                                      const double* __restrict__ rots) {
   extern __shared__ double2 vrow[];
@@ -759,9 +760,7 @@ long long me_jacobi_workspace_size(int m) {
 }
 
 int me_jacobi_set_lds_rows(int rows) {
-  const int prev = g_jac_lds_rows;
-  g_jac_lds_rows = rows < 0 ? 0 : (rows > kMaxLdsRow ? kMaxLdsRow : rows);
-  return prev;
+  return g_jac_lds_rows.exchange(rows < 0 ? 0 : (rows > kMaxLdsRow ? kMaxLdsRow : rows));
 }
 
 int me_jacobi_eig(double* G, double* V, int m, int max_sweeps, double tol, double* eigvals, double* work,
@@ -770,7 +769,7 @@ int me_jacobi_eig(double* G, double* V, int m, int max_sweeps, double tol, doubl
   ME_REQUIRE(m <= 32768, "jacobi_eig: at most 32768 rows");
   const int mm = m + (m & 1);
   const int npairs = mm / 2;
-  const bool lds_v = m <= g_jac_lds_rows;
+  const bool lds_v = m <= g_jac_lds_rows.load();
   double* rots = work;  // kRoundsPerVPass rounds of npairs x 4
   double* part = work + (long long)kRoundsPerVPass * npairs * 4;
   const size_t vshm = (size_t)m * sizeof(double2);
