@@ -462,8 +462,7 @@ struct AdjW {
   static constexpr int RS = R1 * (R2 + 1) + 1;  // exchange row stride (odd)
   static constexpr int OFF_REC = kMaxCol * 8;   // column twiddles at LDS offset 0
   static constexpr int OFF_XCH = OFF_REC + kAdjChunk * 16;
-  static constexpr int OFF_TWR = OFF_XCH + L * RS * 8;
-  static constexpr int OFF_CP = OFF_TWR + NR * 8;
+  static constexpr int OFF_CP = OFF_XCH + L * RS * 8;
   static constexpr int BYTES = OFF_CP + (NR + 1) * 4;
 };
 
@@ -476,7 +475,6 @@ __global__ void __launch_bounds__(AdjW<NR>::NT) k_encode_adj_wave(EncArgs A) {
   float2* twc = (float2*)smem;
   float4* rec = (float4*)(smem + AW::OFF_REC);
   float2* xch = (float2*)(smem + AW::OFF_XCH);
-  float2* twr = (float2*)(smem + AW::OFF_TWR);
   int* cp = (int*)(smem + AW::OFF_CP);
 
   const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
@@ -497,7 +495,6 @@ __global__ void __launch_bounds__(AdjW<NR>::NT) k_encode_adj_wave(EncArgs A) {
   const int mask8 = 8 * n0 - 1;
   const float inv_n0 = 1.0f / (float)n0;
 
-  fill_twiddles<NR, NT>(twr);
   for (int j = tid; j < n0; j += NT) {
     float s, co;
     const int jj = (2 * j > n0) ? j - n0 : j;
@@ -587,8 +584,14 @@ __global__ void __launch_bounds__(AdjW<NR>::NT) k_encode_adj_wave(EncArgs A) {
           a[m1] = c_add(z0, z1);
         }
         RDFT<R1, +1>::run(a);
+        {  // row twiddles from the constant table: t is the wavefront's column phase
+          const int tu = __builtin_amdgcn_readfirstlane(t);
 #pragma unroll
-        for (int k1 = 1; k1 < R1; ++k1) a[k1] = c_mulc(a[k1], twr[(t * k1) % NR]);
+          for (int k1 = 1; k1 < R1; ++k1) {
+            const int j = (tu * k1) % NR;
+            a[k1] = c_mulc(a[k1], make_float2(kTw<NR>.c[j], -kTw<NR>.s[j]));
+          }
+        }
 #pragma unroll
         for (int k1 = 0; k1 < R1; ++k1) xch[r * RS + k1 * (R2 + 1) + t] = a[k1];
       }
