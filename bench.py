@@ -129,6 +129,14 @@ def decode_bytes(pb, s0, s1):
     return float((V + pb.ncoils * V + 8 * pb.ncoils * K).sum())
 
 
+def transform_bytes(pb, B):
+    """Algorithmic bytes of one forward rigid transform of a single image
+    broadcast to B shots (me_rigid_transform mode 0, shared input, 10 line
+    passes): the first pass reads the image once and writes B volumes, the
+    other nine read and write B volumes each."""
+    return float(pb.vol * 8 * (19 * B + 1))
+
+
 def cpu_baseline(shape, ncoils, nshots, n_cg):
     """Times the oracle's complex64 restatement (dense masks, 3-D FFTs) on one
     sample of each unit at the full shape and composes one outer iteration."""
@@ -444,16 +452,17 @@ def main():
         log(f"warmup step {i + 1}/{args.warmup} done")
     pb = app.problem
     timer = KernelTimer(torch)
-    orig_encode, orig_decode = pb._encode, pb._decode
+    orig_encode, orig_decode, orig_rigid = pb._encode, pb._decode, pb._rigid
     pb._encode = timer.wrap("encode_forward", orig_encode)
     pb._decode = timer.wrap("encode_adjoint", orig_decode)
+    pb._rigid = timer.wrap("rigid", orig_rigid)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    pb._encode, pb._decode = orig_encode, orig_decode
+    pb._encode, pb._decode, pb._rigid = orig_encode, orig_decode, orig_rigid
     log(f"timed {args.steps} steps in {t1 - t0:.3f}s")
     ms = (t1 - t0) * 1e3 / args.steps
     # dominant kernel: the gathered forward encoding (k_encode_fwd)
@@ -478,6 +487,21 @@ def main():
         roof_adj = dict(bound="hbm", achieved=round(aach, 1), peak=PEAK_HBM_GBS, unit="GB/s",
                         frac=round(aach / PEAK_HBM_GBS, 4), kernel="k_encode_adj_wave", launches=len(adurs),
                         avg_launch_ms=round(1e3 * float(np.mean(adurs)), 3), bytes_per_launch=float(np.mean(abyts)))
+    # the line passes: forward transforms of one image to the shot batch (T p
+    # in every normal operator), 10 passes per call
+    tbyts, tdurs = [], []
+    for (s, e, a, k), d in zip(timer.events.get("rigid", []), timer.durations("rigid")):
+        x_bs, theta, mode = a[1], a[3], a[4]
+        if mode == 0 and x_bs == 0 and not k.get("flags", 0) and k.get("x_parts", 1) == 1 and len(a) <= 5:
+            tbyts.append(transform_bytes(pb, theta.shape[0]))
+            tdurs.append(d)
+    roof_tr = None
+    if tdurs:
+        tach = float(np.sum(tbyts) / np.sum(tdurs) / 1e9)
+        roof_tr = dict(bound="hbm", achieved=round(tach, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                       frac=round(tach / PEAK_HBM_GBS, 4), kernel="k_linepass (10 passes per transform)",
+                       launches=len(tdurs), avg_launch_ms=round(1e3 * float(np.mean(tdurs)), 3),
+                       bytes_per_launch=float(np.mean(tbyts)))
     res = {
         "metric": "joint-recon outer iters/s at 256x256x192, 32 coils, 64 shots",
         "value": round(args.steps / (t1 - t0), 4),
@@ -499,6 +523,7 @@ def main():
                    "line_tuning": None if tuning is None else list(tuning[0])},
         "roofline": roof,
         "roofline_adjoint": roof_adj,
+        "roofline_transform": roof_tr,
     }
     if not args.no_cpu_baseline:
         log("timing the CPU baseline sample")
