@@ -59,6 +59,10 @@ gradient_entropy                constant volume and single-axis ramp: counts
                                 and entropy in closed form
 error_volume                    hand-computed |a - b| and norm
 shot_masks_from_counters        hand-built acquisition order, remainder drop
+JointRecon, pyramid_reconstruc- compositions of the pinned functions above
+tion, generate_corrupted_kspace (CG, LM, mean re-registration, objective;
+                                levels of resampled data; rigid_apply then the
+                                masked forward encoding per shot)
 ==============================  ===============================================
 """
 from __future__ import annotations
