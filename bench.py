@@ -492,7 +492,8 @@ def main():
     tbyts, tdurs = [], []
     for (s, e, a, k), d in zip(timer.events.get("rigid", []), timer.durations("rigid")):
         x_bs, theta, mode = a[1], a[3], a[4]
-        if mode == 0 and x_bs == 0 and not k.get("flags", 0) and k.get("x_parts", 1) == 1 and len(a) <= 5:
+        if (mode == 0 and x_bs == 0 and not k.get("flags", 0) and k.get("x_parts", 1) == 1 and len(a) <= 5
+                and theta.shape[0] == pb.batch):  # full shot batches (not the one-image re-registration)
             tbyts.append(transform_bytes(pb, theta.shape[0]))
             tdurs.append(d)
     roof_tr = None
