@@ -104,6 +104,25 @@ def test_anisotropic_voxels(me, voxel):
         assert _rel(dT.apply(xt, p_idx), O.rigid_deriv(x64, p, okg, orkg, p_idx)) < TOL
 
 
+@pytest.mark.parametrize("shape", [(32, 24, 48), (64, 64, 48)])
+def test_large_motion(me, shape):
+    """Rotations up to 60 degrees and translations up to 10 voxels: the phase
+    arguments are reduced in FP64 before the FP32 sincos, so accuracy does
+    not degrade with the angle or the shift."""
+    p = np.array([9.5, -7.25, 10.0, np.radians(35), np.radians(-60), np.radians(48)])
+    x = _vol(shape, 6)
+    kg, rkg = me.compute_transform_grids_voxel(shape, (1, 1, 1))
+    okg, orkg = O.compute_transform_grids_voxel(shape, (1, 1, 1))
+    xt = torch.as_tensor(x, device="cuda")
+    x64 = x.astype(np.complex128)
+    T = me.RigidTransformCudaOptimzied(p, kg, rkg)
+    assert _rel(T.apply(xt), O.rigid_apply(x64, p, okg, orkg)) < TOL
+    assert _rel(T.adjoint(xt), O.rigid_adjoint(x64, p, okg, orkg)) < TOL
+    dT = me.RigidTransformDerivativeCuda(shape, p, kg, rkg)
+    for p_idx in (0, 3, 5):
+        assert _rel(dT.apply(xt, p_idx), O.rigid_deriv(x64, p, okg, orkg, p_idx)) < TOL
+
+
 def test_apply_out_argument_quirk(me):
     """transform.py:265: with out=..., apply transforms out's contents."""
     shape = (16, 12, 8)
