@@ -18,7 +18,9 @@ class JointRecon:
     the maps.  ``run()`` returns (fftshift(x * max_norm), t) and fills
     ``transform_history`` (numpy) and ``objective_history`` (numpy).
     ``compute_objective=False`` skips the per-iteration objective (one extra
-    forward encoding, joint_recon.py:102) for timing runs.
+    forward encoding, joint_recon.py:102) for timing runs.  ``t``, ``damp`` and
+    ``convergence_flags`` are the device state; ``x`` is the image in the
+    library's plane-major layout (``image()`` gives the reference layout).
     """
 
     def __init__(self, ksp, mps, shot_mask, kgrid, rkgrid, t0=None, max_cg_iter=3, max_nm_iter=1,
@@ -40,6 +42,17 @@ class JointRecon:
     @property
     def t(self):
         return self.problem.theta
+
+    @property
+    def damp(self):
+        """LM damping per shot (joint_recon.py:81), FP64 device tensor."""
+        return self.problem.damp
+
+    @property
+    def convergence_flags(self):
+        """Per-shot flags of transform_estimation.py:139-140 (written, never
+        read, as in the reference), uint8 device tensor."""
+        return self.problem.flags
 
     def step(self, prev_objective=None):
         """One outer iteration (joint_recon.py:92-100).

@@ -33,4 +33,4 @@ def test_joint_recon_midsize_matches_oracle(me):
     d = np.abs(t_got.cpu().numpy() - t_ref)
     assert max(d[:, :3].max(), np.degrees(d[:, 3:]).max()) < 1e-3
     assert np.allclose(app.objective_history, ref.objective_history, rtol=1e-4, atol=0)
-    assert np.array_equal(app.problem.damp.cpu().numpy(), ref.damp)
+    assert np.array_equal(app.damp.cpu().numpy(), ref.damp)
