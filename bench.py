@@ -16,6 +16,7 @@ baseline (the complex64 NumPy/SciPy restatement timed on a bounded sample).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -52,6 +53,11 @@ def parse():
                     help="calibration workload instead of the headline: noise whitening, ESPIRiT (24^3 "
                          "calibration region, kernel 5^3) and coil compression (perc 0.95) at 256x256x192 with 32 "
                          "coils; per-stage GPU times and a CPU baseline on bounded samples (prints its own JSON line)")
+    ap.add_argument("--plain", action="store_true",
+                    help="comparison workload instead of the headline: the normal operator (sum over 64 shots of "
+                         "T^H E^H E T) and the forward E T at 256x256x192 with 32 coils, through the library and "
+                         "through a plain torch.fft restatement of the reference's algorithm on the same GPU (dense "
+                         "masks, 3-D FFTs of every coil, 1-D FFT shears); prints its own JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (three times), the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
@@ -250,6 +256,110 @@ def grad_bench(args, torch, me, dev):
     print(json.dumps(out), flush=True)
 
 
+def plain_transform(torch, x, th, kg, rkg, adjoint=False):
+    """The reference's FFT-shear rigid transform (transform.py:263-377) written
+    plainly with torch.fft: rotations about x, y, z of three 1-D Fourier shears
+    each (tan, sin, tan), then a 3-D Fourier translation; the adjoint applies
+    the conjugated translation first and the conjugated shears in reverse."""
+    q1, q2, q3, t1, t2, t3 = [float(v) for v in th]
+    steps = [(t1, rkg["tan"]["x"], rkg["sin"]["x"], 1, 2), (t2, rkg["tan"]["y"], rkg["sin"]["y"], 2, 0),
+             (t3, rkg["tan"]["z"], rkg["sin"]["z"], 0, 1)]
+    sgn = -1.0 if adjoint else 1.0
+
+    def shear(a, ax, ph):
+        return torch.fft.ifft(torch.fft.fft(a, dim=ax) * ph, dim=ax)
+
+    def translate(a):
+        U = torch.exp(-1j * sgn * (q1 * kg["x"] + q2 * kg["y"] + q3 * kg["z"])).to(torch.complex64)
+        return torch.fft.ifftn(torch.fft.fftn(a) * U)
+
+    if adjoint:
+        x = translate(x)
+    for th_s, rkt, rks, at, as_ in (steps[::-1] if adjoint else steps):
+        vt = torch.exp(1j * sgn * math.tan(th_s / 2) * rkt).to(torch.complex64)
+        vs = torch.exp(-1j * sgn * math.sin(th_s) * rks).to(torch.complex64)
+        x = shear(shear(shear(x, at, vt), as_, vs), at, vt)
+    return x if adjoint else translate(x)
+
+
+def plain_bench(args, torch, me, dev):
+    """Library against a plain torch.fft restatement of the reference's
+    per-shot loop (image_estimation.py:106-129: transform, coil maps, 3-D FFT
+    of every coil, dense mask, inverse 3-D FFT, conj(maps) coil sum, adjoint
+    transform) on the same GPU, same inputs, complex64 both."""
+    import numpy as np
+
+    cfg = me.simulation.CONFIGS["c5"]
+    shape, ncoils, nshots = cfg["shape"], cfg["coils"], cfg["nshots"]
+    mps = me.simulation.coil_maps(shape, ncoils, seed=1, device=dev)
+    A, _ = me.simulation.disorder_sampling(shape, cfg["tile"], cfg["accel"], nshots=nshots)
+    kg, rkg = me.compute_transform_grids_voxel(shape, (1, 1, 1))
+    pb = me.encoding.SenseProblem(torch.zeros_like(mps), mps, A, kg, rkg, centered=True)
+    rng = np.random.default_rng(11)
+    theta = np.concatenate([rng.uniform(-2, 2, (nshots, 3)), np.radians(rng.uniform(-3, 3, (nshots, 3)))], 1)
+    pb.theta.copy_(torch.as_tensor(theta))
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(12)
+    x = torch.randn(shape, dtype=torch.complex64, device=dev, generator=gen)
+    # the plain path's operands in the reference's uncentred layout
+    mps_u = torch.fft.ifftshift(mps, dim=(1, 2, 3)).contiguous()
+    del mps
+    masks = torch.as_tensor(np.fft.ifftshift(A, axes=(1, 2)), device=dev).reshape(nshots, shape[0], shape[1], 1)
+    M = (mps_u.abs().square().sum(0) > 0.1).to(torch.complex64)
+    tkg, trkg = me.compute_transform_grids_voxel(shape, (1, 1, 1), xp=torch)
+
+    def plain_forward():
+        k = None
+        for s in range(nshots):
+            Tx = plain_transform(torch, x, theta[s], tkg, trkg)
+            k = torch.fft.fftn(mps_u * Tx, dim=(1, 2, 3), norm="ortho") * masks[s]
+        return k
+
+    def plain_normal():
+        out = torch.zeros_like(x)
+        for s in range(nshots):
+            Tx = plain_transform(torch, x, theta[s], tkg, trkg)
+            k = torch.fft.fftn(mps_u * Tx, dim=(1, 2, 3), norm="ortho") * masks[s]
+            t = torch.fft.ifftn(k, dim=(1, 2, 3), norm="ortho")
+            out += plain_transform(torch, (mps_u.conj() * t).sum(0), theta[s], tkg, trkg, adjoint=True)
+        return out * M
+
+    xp = pb.image_to_plane(x)
+    outp = pb.zeros_image()
+
+    def ours_normal():
+        outp.zero_()
+        return pb.normal(xp, outp)
+
+    res = {}
+    for name, fn in (("ours_forward", lambda: pb.forward(xp)), ("ours_normal", ours_normal),
+                     ("plain_forward", plain_forward), ("plain_normal", plain_normal)):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t_start) * 1e3 / args.steps
+        log(f"{name}: {res[name]:.2f} ms")
+    ref = plain_normal()
+    got = pb.image_from_plane(ours_normal())
+    rel = float(torch.linalg.vector_norm(got - ref) / torch.linalg.vector_norm(ref))
+    out = {
+        "metric": "normal operator applications per second at 256x256x192, 32 coils, 64 shots",
+        "value": round(1e3 / res["ours_normal"], 3), "unit": "applications/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": True, "dtype": "c64", "data": "synthetic",
+        "config": {"workload": f"{shape[0]}x{shape[1]}x{shape[2]} complex64, {ncoils} coils, {nshots} DISORDER "
+                               f"shots, random image and motion (+-2 voxels, +-3 degrees)"},
+        "ms": {k: round(v, 2) for k, v in res.items()},
+        "speedup_over_plain": {"normal": round(res["plain_normal"] / res["ours_normal"], 2),
+                               "forward": round(res["plain_forward"] / res["ours_forward"], 2)},
+        "normal_rel_l2_vs_plain": rel,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def batch_bench(args, torch, me, dev):
     """Batched vs one-at-a-time joint reconstruction of uneven volumes."""
     specs = me.simulation.batch_specs(24, seed=4)
@@ -419,6 +529,9 @@ def main():
         return
     if args.calib:
         calib_bench(args, torch, me, dev)
+        return
+    if args.plain:
+        plain_bench(args, torch, me, dev)
         return
     t_setup = time.perf_counter()
     log(f"generating synthetic data for {args.config}")

@@ -2,7 +2,8 @@
 32 coils, DISORDER tile 16 x 16, acceleration 2, 64 shots; 4 shots used),
 where the FP64 oracle is too slow to run: unitarity and adjoint identities
 of the transform, the encoding and the normal operator, and the round trip
-T^H T = I."""
+T^H T = I; and the encoding and normal operator against a plain torch.fft
+restatement of the reference's algorithm on the same GPU."""
 import numpy as np
 import pytest
 import torch
@@ -34,7 +35,7 @@ def full(me):
     pb.theta.copy_(th)
     x = torch.randn(pb.vol, dtype=torch.complex64, device=dev, generator=g)
     y = torch.randn(pb.vol, dtype=torch.complex64, device=dev, generator=g)
-    return dict(pb=pb, th=th, x=x, y=y, g=g)
+    return dict(pb=pb, th=th, x=x, y=y, g=g, A=A)
 
 
 def test_transform_unitary_and_round_trip(me, full):
@@ -85,6 +86,42 @@ def test_normal_operator_hermitian_on_support(me, full):
     rhs = complex(torch.vdot(xm, Ay))
     assert abs(lhs - rhs) / abs(lhs) < 1e-5
     assert float(torch.vdot(xm, Ax).real) > 0
+
+
+def test_forward_and_normal_match_plain_torch_restatement(me, full):
+    """The gathered encoding E_s T_s x and the normal operator against the
+    reference's algorithm restated plainly with torch.fft on the same GPU
+    (bench.py's comparison leg): 1-D FFT shears and a 3-D FFT translation,
+    dense masks, 3-D FFTs of every coil.  complex64 both, relative L2 1e-4."""
+    from bench import plain_transform
+
+    pb, th, x = full["pb"], full["th"], full["x"]
+    dev = x.device
+    n2 = SHAPE[2]
+    mps_u = torch.fft.ifftshift(me.simulation.coil_maps(SHAPE, NC, seed=1, device=dev), dim=(1, 2, 3))
+    A_u = np.fft.ifftshift(full["A"], axes=(1, 2)).reshape(NS, SHAPE[0], SHAPE[1], 1)
+    tkg, trkg = me.compute_transform_grids_voxel(SHAPE, (1, 1, 1), xp=torch)
+    xi = pb.image_from_plane(x)
+    theta = th.cpu().numpy()
+    Y = pb.forward(x)
+    pl = pb.plan
+    ref_n = torch.zeros_like(xi)
+    for s in range(NS):
+        Tx = plain_transform(torch, xi, theta[s], tkg, trkg)
+        k = torch.fft.fftn(mps_u * Tx, dim=(1, 2, 3), norm="ortho") * torch.as_tensor(A_u[s], device=dev)
+        hyb = torch.fft.ifft(k, dim=3, norm="ortho")
+        o, c = int(pl.off_h[s]), int(pl.cnt_h[s])
+        k0 = torch.as_tensor(pl.k0_h[o:o + c], dtype=torch.long, device=dev)
+        k1 = torch.as_tensor(pl.k1_h[o:o + c], dtype=torch.long, device=dev)
+        got = Y[o * n2 * NC:(o + c) * n2 * NC].reshape(n2, NC, c).permute(1, 2, 0)
+        assert _rel(got, hyb[:, k0, k1, :]) < 1e-4, s
+        del hyb
+        t = torch.fft.ifftn(k, dim=(1, 2, 3), norm="ortho")
+        ref_n += plain_transform(torch, (mps_u.conj() * t).sum(0), theta[s], tkg, trkg, adjoint=True)
+    ref_n *= (mps_u.abs().square().sum(0) > 0.1)
+    out = pb.zeros_image()
+    pb.normal(x, out)
+    assert _rel(pb.image_from_plane(out), ref_n) < 1e-4
 
 
 def test_espirit_routes_agree_at_full_size(me):
