@@ -571,17 +571,20 @@ __global__ void __launch_bounds__(AdjW<NR>::NT) k_encode_adj_wave(EncArgs A) {
             w = twc[mulmod(k, xr, n0, -1, inv_n0)];
           z = c_fmac(w, make_float2(q.x, q.y), z);  // twc holds e^{-2 pi i j/n0}: conjugate
         };
+        // one accumulator per column, zeroed once for all columns up front
+        // (per-column zeroing inside the loop nest is rematerialised on both
+        // sides of every loop guard)
+#pragma unroll
+        for (int m1 = 0; m1 < R1; ++m1) a[m1] = make_float2(0.f, 0.f);
 #pragma unroll
         for (int m1 = 0; m1 < R1; ++m1) {
-          float2 z0 = make_float2(0.f, 0.f), z1 = make_float2(0.f, 0.f);
           int o = lo16[m1];
           const int oe = hi16[m1];
           for (; o + 16 < oe; o += 32) {
-            term(z0, o);
-            term(z1, o + 16);
+            term(a[m1], o);
+            term(a[m1], o + 16);
           }
-          if (o < oe) term(z0, o);
-          a[m1] = c_add(z0, z1);
+          if (o < oe) term(a[m1], o);
         }
         RDFT<R1, +1>::run(a);
         {  // row twiddles from the constant table: t is the wavefront's column phase
