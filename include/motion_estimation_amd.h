@@ -260,7 +260,7 @@ int me_sum_batch(const void* x, long long x_bstride, int nbatch, long long n, vo
 /* y += a x for complex vectors and a real scalar. */
 int me_axpy_real(void* y, double a, const void* x, long long n, hipStream_t stream);
 
-/* out = a - b (complex, n values). *//* out = a - b (complex, n values). */
+/* out = a - b (complex, n values). */
 int me_csub(void* out, const void* a, const void* b, long long n, hipStream_t stream);
 
 /* x *= mask (uint8) (joint_recon.py:100). */
