@@ -172,6 +172,23 @@ int me_encode_forward(const void* u, long long u_bstride, const void* maps, int 
                       const int* samp_k0, const int* samp_k1, const int* shot_off, const int* shot_cnt, int max_cnt,
                       void* y, const void* y_meas, double* partials, double* norms, int flags, hipStream_t stream);
 
+/* Allocation-free variant of me_encode_forward.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as me_encode_forward without `partials`: the
+ * per-workgroup FP64 norm partials live in a library-owned static device
+ * array of ME_ENCODE_STATIC_PARTIALS doubles (8 MiB, one per GPU, held until
+ * the process ends).  Needs nbatch * me_encode_partials_size(N2, ncoils,
+ * max_cnt, N1) <= ME_ENCODE_STATIC_PARTIALS when norms is given, else it
+ * fails with an error before any device work.  `stream` must belong to the
+ * current device.  Every call on one GPU shares that GPU's array: calls that
+ * overlap (concurrent streams) race.  Calls no hipMalloc or hipFree. */
+#define ME_ENCODE_STATIC_PARTIALS (1 << 20)
+int me_encode_forward_static(const void* u, long long u_bstride, const void* maps, int ncoils, int nbatch,
+                             const int* n, const int* samp_k0, const int* samp_k1, const int* shot_off,
+                             const int* shot_cnt, int max_cnt, void* y, const void* y_meas, double* norms, int flags,
+                             hipStream_t stream);
+
 /* Number of partial images per shot written by me_encode_adjoint (1: the
  * adjoint sums all coils itself). */
 int me_encode_coil_groups(int ncoils);

@@ -32,6 +32,7 @@ _SIGS = {
                              i32, vp],
     "me_fft_axis": [vp, i64, vp, i64, i32, vp, vp, i32, i32, vp],
     "me_encode_forward": [vp, i64, vp, i32, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, vp],
+    "me_encode_forward_static": [vp, i64, vp, i32, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, vp],
     "me_encode_adjoint": [vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, i64, vp],
     "me_permute_shift": [vp, vp, vp, vp, vp, vp, vp, C.c_float, vp],
     "me_coil_ssq": [vp, i32, i64, C.c_float, C.c_float, vp, vp, vp],

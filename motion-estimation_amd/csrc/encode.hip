@@ -22,6 +22,9 @@
 //
 // Volumes (image u, maps S_c) are in plane-major layout: element (x0, x1, x2)
 // at (x2*N0 + x0)*N1 + x1.  Samples of a shot are sorted by (k1, k0).
+#include <mutex>
+#include <string>
+
 #include "me_device.h"
 #include "me_internal.h"
 #include "../../include/motion_estimation_amd.h"
@@ -801,24 +804,46 @@ int threads_for_row(int n1) {
 }
 }  // namespace
 
-}  // namespace me
+// Norm partials of me_encode_forward_static: a file-scope device array (each
+// GPU holds its own copy until the process ends).  Host code never passes the
+// symbol itself to a kernel: its device address, looked up once per device
+// with hipGetSymbolAddress, is what the kernels receive.  Every entry a call
+// reads (k_sum_partials) is written by that call's k_encode_fwd first, so
+// nothing needs clearing; calls that overlap on one GPU share the array and
+// race.
+constexpr long long kStaticPartials = ME_ENCODE_STATIC_PARTIALS;
+__device__ __align__(256) double scratch[kStaticPartials];
 
-using namespace me;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
 
-extern "C" {
-
-long long me_encode_partials_size(int n2, int ncoils, int max_cnt, int n1) {
-  const int nt = threads_for_row(n1);
-  if (nt == 0 || ncoils <= 0 || n2 <= 0) return 0;
-  const int ncg = (ncoils + kCoilsPerWG - 1) / kCoilsPerWG;
-  const int spt = 2;
-  const int nchunk = max_cnt > 0 ? (max_cnt + spt * nt - 1) / (spt * nt) : 1;
-  return (long long)n2 * ncg * nchunk;
+// Device address of `scratch` on the device that runs `stream` (which must be
+// the current device).
+int static_partials(hipStream_t stream, double** out) {
+  int dev = -1;
+  ME_TRY_HIP(hipGetDevice(&dev));
+  if (stream) {
+    int sdev = -1;
+    ME_TRY_HIP(hipStreamGetDevice(stream, &sdev));
+    ME_REQUIRE(sdev == dev, "encode_forward_static: the stream belongs to device " + std::to_string(sdev) +
+                                ", not the current device " + std::to_string(dev));
+  }
+  ME_REQUIRE(dev >= 0 && dev < kMaxDevices, "encode_forward_static: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    ME_TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ME_REQUIRE(p != nullptr, "encode_forward_static: no device address for the scratch array");
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
 }
 
-int me_encode_forward(const void* u, long long u_bstride, const void* maps, int ncoils, int nbatch, const int* n,
-                      const int* samp_k0, const int* samp_k1, const int* shot_off, const int* shot_cnt, int max_cnt,
-                      void* y, const void* y_meas, double* partials, double* norms, int flags, hipStream_t stream) {
+int encode_forward_impl(const void* u, long long u_bstride, const void* maps, int ncoils, int nbatch, const int* n,
+                        const int* samp_k0, const int* samp_k1, const int* shot_off, const int* shot_cnt, int max_cnt,
+                        void* y, const void* y_meas, double* partials, double* norms, int flags, hipStream_t stream) {
   ME_REQUIRE(u && maps && n && samp_k0 && samp_k1 && shot_off && shot_cnt, "encode_forward: null argument");
   ME_REQUIRE(nbatch >= 1 && nbatch <= 65535 && ncoils >= 1, "encode_forward: bad batch/coil count");
   ME_REQUIRE(n[0] <= kMaxCol, "encode_forward: N0 above 512");
@@ -856,6 +881,47 @@ int me_encode_forward(const void* u, long long u_bstride, const void* maps, int 
     ME_TRY_HIP(hipGetLastError());
   }
   return 0;
+}
+
+}  // namespace me
+
+using namespace me;
+
+extern "C" {
+
+long long me_encode_partials_size(int n2, int ncoils, int max_cnt, int n1) {
+  const int nt = threads_for_row(n1);
+  if (nt == 0 || ncoils <= 0 || n2 <= 0) return 0;
+  const int ncg = (ncoils + kCoilsPerWG - 1) / kCoilsPerWG;
+  const int spt = 2;
+  const int nchunk = max_cnt > 0 ? (max_cnt + spt * nt - 1) / (spt * nt) : 1;
+  return (long long)n2 * ncg * nchunk;
+}
+
+int me_encode_forward(const void* u, long long u_bstride, const void* maps, int ncoils, int nbatch, const int* n,
+                      const int* samp_k0, const int* samp_k1, const int* shot_off, const int* shot_cnt, int max_cnt,
+                      void* y, const void* y_meas, double* partials, double* norms, int flags, hipStream_t stream) {
+  return encode_forward_impl(u, u_bstride, maps, ncoils, nbatch, n, samp_k0, samp_k1, shot_off, shot_cnt, max_cnt, y,
+                             y_meas, partials, norms, flags, stream);
+}
+
+int me_encode_forward_static(const void* u, long long u_bstride, const void* maps, int ncoils, int nbatch,
+                             const int* n, const int* samp_k0, const int* samp_k1, const int* shot_off,
+                             const int* shot_cnt, int max_cnt, void* y, const void* y_meas, double* norms, int flags,
+                             hipStream_t stream) {
+  double* partials = nullptr;
+  if (norms) {
+    ME_REQUIRE(n && nbatch >= 1, "encode_forward_static: bad argument");
+    const long long need = me_encode_partials_size(n[2], ncoils, max_cnt, n[1]) * (long long)nbatch;
+    ME_REQUIRE(need <= kStaticPartials,
+               "encode_forward_static: " + std::to_string(need) + " FP64 norm partials needed, the static scratch "
+               "holds " + std::to_string(kStaticPartials) + " (ME_ENCODE_STATIC_PARTIALS); use me_encode_forward "
+               "with a workspace or fewer shots per call");
+    const int rc = static_partials(stream, &partials);
+    if (rc) return rc;
+  }
+  return encode_forward_impl(u, u_bstride, maps, ncoils, nbatch, n, samp_k0, samp_k1, shot_off, shot_cnt, max_cnt, y,
+                             y_meas, partials, norms, flags, stream);
 }
 
 int me_encode_coil_groups(int ncoils) { return ncoils >= 1 ? 1 : 0; }
