@@ -159,7 +159,7 @@ int me_graph_destroy(void* exec);
 long long me_encode_partials_size(int n2, int ncoils, int max_cnt, int n1);
 
 /* Gathered forward encoding of a batch of shots:
- *   Y_s = M_s F_01 (S_c u_s)      (transform_estimation.py:207-210 without the
+ *   Y_s = M_s F_01 (S_c u_s)      (transform_estimation.py:42-45 without the
  *   readout FFT, which is applied once to the measured data)
  * u: image per shot at u_bstride (0: one image for all shots).  If y_meas is
  * given, Y - y_meas is produced (the residual) unless flags bit 0
@@ -287,13 +287,13 @@ int me_mask_mul(void* x, const unsigned char* mask, long long n, hipStream_t str
 long long me_lm_partials_size(int nbatch);
 
 /* Per shot: stats[28] = {||r||^2, Re J_p^H r (6), Re J_i^H J_j (i<=j, 21)} over
- * its gathered samples in FP64 (transform_estimation.py:212-233).  r and the
+ * its gathered samples in FP64 (transform_estimation.py:47-68).  r and the
  * columns use the gathered layout; column p at J + p*jstride; per_sample =
  * N2*C values per sample. */
 int me_lm_gram(const void* r, const void* J, long long jstride, const int* shot_off, const int* shot_cnt,
                int per_sample, int nbatch, double* partials, double* stats, hipStream_t stream);
 
-/* (Gram + damp I) delta = g, cand = theta - delta (transform_estimation.py:236, 290). */
+/* (Gram + damp I) delta = g, cand = theta - delta (transform_estimation.py:71, 125). */
 int me_lm_solve(const double* stats, const double* damp, const double* theta, int theta_stride, double* cand,
                 double* delta, int nbatch, hipStream_t stream);
 
@@ -311,12 +311,12 @@ int me_lm_ga_solve(const double* stats, const double* stats_dir, const double* d
 
 /* Accept iff e1 < e0: theta = cand, damp = max(damp/5, 1e-4), flags set if
  * |theta - cand| < tol; else damp = min(1.5 damp, 1e16)
- * (transform_estimation.py:303-309).  accepted (nullable) records decisions. */
+ * (transform_estimation.py:138-144).  accepted (nullable) records decisions. */
 int me_lm_accept(const double* stats, const double* e1, double* theta, int theta_stride, const double* cand,
                  double* damp, unsigned char* flags, unsigned char* accepted, double tol, int nbatch,
                  hipStream_t stream);
 
-/* mean[6] = mean over shots of theta; theta -= mean (transform_estimation.py:312-315). */
+/* mean[6] = mean over shots of theta; theta -= mean (transform_estimation.py:147-150). */
 int me_lm_mean(double* theta, int theta_stride, int nshots, double* mean, hipStream_t stream);
 
 /* ---------------------------------------------------------------- host

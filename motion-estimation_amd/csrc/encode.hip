@@ -2,7 +2,7 @@
 // (k0, k1, x2) space.
 //
 // The reference computes, per shot s and coil c (image_estimation.py:94-128,
-// transform_estimation.py:207-210, joint_recon.py:428-429)
+// transform_estimation.py:42-45, joint_recon.py:112-113)
 //     E_s u = M_s * fftn(S_c * u)   over all three axes, with a dense boolean
 // mask M_s(k0, k1) broadcast along the readout axis 2.  Because the mask does
 // not depend on k2 and the 3-D FFT is orthonormal and separable, the readout

@@ -9,7 +9,7 @@
 // update when sqrt|<r, z>| < tol, image_estimation.py:156-157) sets a device
 // flag that turns every later update into a no-op.
 //
-// LM (transform_estimation.py:204-315): per shot, e0 = ||r||^2, g = Re J^H r
+// LM (transform_estimation.py:39-150): per shot, e0 = ||r||^2, g = Re J^H r
 // and the Gram matrix Re J^H J over the gathered samples (FP64), the damped
 // 6x6 solve in FP64 (Gaussian elimination with partial pivoting), the
 // candidate theta - delta, accept/reject with the reference's damping rule,
@@ -294,8 +294,8 @@ __global__ void k_lm_gram_finalize(const double* __restrict__ partial, int nblk,
   stats[(long long)b * kStats + k] = s;
 }
 
-// H delta = g with H = Gram + damp*I (transform_estimation.py:225-236), then
-// candidate = theta - delta (transform_estimation.py:290).  Singular systems
+// H delta = g with H = Gram + damp*I (transform_estimation.py:60-71), then
+// candidate = theta - delta (transform_estimation.py:125).  Singular systems
 // produce non-finite deltas, as a failed LU would.
 // d = (Gram + damp I)^-1 rhs for the Gram matrix packed in a stats row
 // (Gaussian elimination with partial pivoting, FP64).
@@ -406,7 +406,7 @@ __global__ void k_lm_ga_solve(const double* __restrict__ stats, const double* __
     cand[(long long)b * 6 + p] = theta[(long long)b * th_stride + p] + v[p] + (take ? 0.5 * a[p] : 0.0);
 }
 
-// transform_estimation.py:303-309
+// transform_estimation.py:138-144
 __global__ void k_lm_accept(const double* __restrict__ stats, const double* __restrict__ e1, double* __restrict__ theta,
                             int th_stride, const double* __restrict__ cand, double* __restrict__ damp,
                             unsigned char* __restrict__ flags, unsigned char* __restrict__ accepted, double tol,
@@ -427,7 +427,7 @@ __global__ void k_lm_accept(const double* __restrict__ stats, const double* __re
   if (accepted) accepted[b] = acc ? 1 : 0;
 }
 
-// mean over shots (fixed order) and subtraction (transform_estimation.py:312-315)
+// mean over shots (fixed order) and subtraction (transform_estimation.py:147-150)
 __global__ void k_lm_mean(double* __restrict__ theta, int th_stride, int nshots, double* __restrict__ mean) {
   __shared__ double m[6];
   if (threadIdx.x < 6) {

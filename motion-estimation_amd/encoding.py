@@ -405,7 +405,7 @@ class SenseProblem:
         return float(self.objective_per_shot(x, theta).sum().item())
 
     def lm_step(self, x, tol=1e-6, record=None, tracked=False, use_ga=False, out=None):
-        """One LM update of every shot (transform_estimation.py:204-315).
+        """One LM update of every shot (transform_estimation.py:39-150).
         Updates self.theta/damp/flags in place, returns T(mean) x (written to
         ``out`` when given, which may be x itself).
 

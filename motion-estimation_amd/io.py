@@ -9,7 +9,7 @@ extract_from_dat.py:6-60, simulation.py:476-493).
 * ``prepare_real_data``: the calibration chain of recon.py:130-150 on the
   GPU (noise whitening, ESPIRiT from the reference scan cropped to 24^3,
   coil compression).
-* ``reconstruct_real_data``: the rest of recon.py:129-193 from those arrays
+* ``reconstruct_real_data``: the rest of recon.py:130-172 from those arrays
   (SENSE image, pyramid motion correction over the counter masks, the pre/
   and recon/ outputs).
 * ``save_nifti`` / ``load_nifti``: NIfTI-1 single files (.nii or .nii.gz)
@@ -101,7 +101,7 @@ def prepare_real_data(ksp, ref, noise, lines=None, partitions=None, n_shots=None
 
 def reconstruct_real_data(ksp, ref, noise, lines, partitions, n_shots, out_dir=None, sampling="disorder", iters=1000,
                           calib_size=24, sense_iter=100, pyramid_kwargs=None):
-    """recon.py:129-193 from the arrays the TWIX extraction yields (ksp and
+    """recon.py:130-172 from the arrays the TWIX extraction yields (ksp and
     ref as (coils, par, lin, col), noise coils first, the cLin / cPar
     counters of the imaging readouts in acquisition order):
 

@@ -395,8 +395,8 @@ def _vdot_real(a, b, prec):
 
 
 def encode_forward(x, mps, mask_s, params, kgrid, rkgrid, prec="f64"):
-    """mask_s * fftn(mps * T x)  (transform_estimation.py:207-210 without the
-    subtraction; joint_recon.py:428-429)."""
+    """mask_s * fftn(mps * T x)  (transform_estimation.py:42-45 without the
+    subtraction; joint_recon.py:112-113)."""
     cdt = _cdt(prec)
     Tx = rigid_apply(x, params, kgrid, rkgrid, prec=prec)
     return (_fftn((mps * Tx).astype(cdt, copy=False), (-3, -2, -1), WORKERS) * mask_s).astype(cdt, copy=False)
@@ -469,7 +469,7 @@ def estimate_image_cg(ksp, mps, shot_mask, transforms, kgrid, rkgrid, P, M, x0=N
 # a9-a10: LM motion update  (transform_estimation.py:7-152, use_ga=False)
 # ---------------------------------------------------------------------------
 def shot_residual(ksp, mps, mask_s, image, params, kgrid, rkgrid, prec="f64"):
-    """transform_estimation.py:206-210."""
+    """transform_estimation.py:41-45."""
     cdt = _cdt(prec)
     Tx = rigid_apply(image, params, kgrid, rkgrid, prec=prec)
     r = _fftn((mps * Tx).astype(cdt, copy=False), (-3, -2, -1), WORKERS)
@@ -477,7 +477,7 @@ def shot_residual(ksp, mps, mask_s, image, params, kgrid, rkgrid, prec="f64"):
 
 
 def shot_jacobian(mps, mask_s, image, params, kgrid, rkgrid, prec="f64"):
-    """transform_estimation.py:216-220: six columns mask*fftn(mps*dT_p x)."""
+    """transform_estimation.py:51-55: six columns mask*fftn(mps*dT_p x)."""
     cdt = _cdt(prec)
     cols = []
     for p_idx in range(6):
@@ -487,7 +487,7 @@ def shot_jacobian(mps, mask_s, image, params, kgrid, rkgrid, prec="f64"):
 
 
 def lm_normal_equations(residual, cols, damp_s, prec="f64"):
-    """transform_estimation.py:212-233: e0, gradient, damped Gram matrix."""
+    """transform_estimation.py:47-68: e0, gradient, damped Gram matrix."""
     e0 = _vdot_real(residual, residual, prec)
     g = np.array([_vdot_real(residual, c, prec) for c in cols], dtype=np.float64)
     H = np.zeros((6, 6), dtype=np.float64)
@@ -536,17 +536,17 @@ def estimate_transform(ksp, mps, shot_mask, image, kgrid, rkgrid, damp, converge
         res = shot_residual(ksp, mps, shot_mask[s], image, transforms[s], kgrid, rkgrid, prec)
         cols = shot_jacobian(mps, shot_mask[s], image, transforms[s], kgrid, rkgrid, prec)
         e0, g, H = lm_normal_equations(res, cols, damp[s], prec)
-        delta = np.linalg.solve(H, g)  # transform_estimation.py:236
+        delta = np.linalg.solve(H, g)  # transform_estimation.py:71
         if use_ga:
             v = -delta
             jpv = ga_direction(mps, shot_mask[s], image, transforms[s], cols, v, kgrid, rkgrid, prec)
             nxt = transforms[s] + ga_step(H, cols, jpv, v, prec)
         else:
-            nxt = transforms[s] - delta  # transform_estimation.py:290
+            nxt = transforms[s] - delta  # transform_estimation.py:125
         res1 = shot_residual(ksp, mps, shot_mask[s], image, nxt, kgrid, rkgrid, prec)
         e1 = _vdot_real(res1, res1, prec)
         acc = e1 < e0
-        if acc:  # transform_estimation.py:303-309
+        if acc:  # transform_estimation.py:138-144
             if np.all(np.abs(transforms[s] - nxt) < tol):
                 convergence_flags[s] = True
             transforms[s] = nxt.copy()
@@ -555,7 +555,7 @@ def estimate_transform(ksp, mps, shot_mask, image, kgrid, rkgrid, damp, converge
             damp[s] = min(damp[s] * 1.5, 1e16)
         if trace is not None:
             trace.append((e0, e1, bool(acc), delta.copy()))
-    mean_t = np.mean(transforms, axis=0)  # transform_estimation.py:312-315
+    mean_t = np.mean(transforms, axis=0)  # transform_estimation.py:147-150
     out = rigid_apply(image, mean_t, kgrid, rkgrid, prec=prec)
     transforms -= mean_t
     return out

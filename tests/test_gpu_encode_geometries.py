@@ -1,5 +1,5 @@
 """Encoding parity against the FP64 oracle on geometries that select the other
-kernel variants (image_estimation.py:94-128, transform_estimation.py:207-210):
+kernel variants (image_estimation.py:94-128, transform_estimation.py:42-45):
 
 * column lengths N0 that are not powers of two (the adjoint's twiddle index
   by multiply-and-correct instead of a mask);

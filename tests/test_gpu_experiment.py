@@ -5,7 +5,7 @@ baseline and the pyramid.  Small size (32 x 32 x 8, 4 coils), one joint
 iteration per level, 2 levels and 20 final CG iterations (the pyramid test's
 regime).  Relative L2 1e-4 on k-space and images, 1e-3 (degrees, voxels) on
 motion; the written NIfTI and npz files are read back.  The real-data driver
-(recon.py:129-193) likewise, from synthetic scanner arrays."""
+(recon.py:130-172) likewise, from synthetic scanner arrays."""
 import os
 
 import numpy as np

@@ -399,7 +399,7 @@ def test_ga_step_gate_and_half_step():
 
 
 def test_lm_damping_rules():
-    """transform_estimation.py:303-309: accept -> damp/5 (floor 1e-4),
+    """transform_estimation.py:138-144: accept -> damp/5 (floor 1e-4),
     reject -> damp*1.5 (cap 1e16)."""
     shape = (8, 8, 4)
     mps, mask, _, _ = _small_problem(shape, 2, 1, seed=7)
