@@ -40,8 +40,13 @@ estimate_transform              6x6 solve vs numpy.linalg.solve on explicit
 ga_direction / ga_step          closed form of J'v for translation directions
                                 (-(k.v)^2 in Fourier); gate and half-step by
                                 hand on a constructed system
-make_disorder_order             bit-exact stdlib ``random`` (the reference uses
-                                ``random.Random``); tile/union invariants
+make_disorder_order             orders written by the reference's own
+                                make_disorder_order for 7 tile settings
+                                (tests/golden/disorder_orders.npz), bit-exact;
+                                tile/union invariants; first tile equals a
+                                direct stdlib ``random.shuffle``
+disorder_masks                  hand-built masks including the dropped
+                                remainder of the order
 pyramid resampling              SigPy centred fft/resize restated; crop KATs
 noise_whiten                    inv(W^H) is lower triangular with a positive
                                 diagonal and squares to cov + reg I
@@ -59,10 +64,22 @@ gradient_entropy                constant volume and single-axis ramp: counts
                                 and entropy in closed form
 error_volume                    hand-computed |a - b| and norm
 shot_masks_from_counters        hand-built acquisition order, remainder drop
-JointRecon, pyramid_reconstruc- compositions of the pinned functions above
-tion, generate_corrupted_kspace (CG, LM, mean re-registration, objective;
-                                levels of resampled data; rigid_apply then the
-                                masked forward encoding per shot)
+generate_corrupted_kspace       zero motion: masked brute-force centred DFT of
+                                S*gt; integer per-shot translations: each
+                                shot's circular shift on its own samples
+JointRecon                      identity operator (one CG step returns gt,
+                                max_norm = max|S gt|, objective 0); single-coil
+                                zero-filled projection (ksp/mask shifts);
+                                objective of data in the null space of each
+                                E_s^H = ||d||^2 / max_norm^2; P and M values of
+                                a piecewise coil sum; x zero off M after the
+                                mean re-registration
+pyramid_reconstruction          per-level JointRecon replaced by a recorder:
+                                level shapes, grids, iteration counts, t0 of
+                                each level (carried unscaled), final CG with
+                                the last level's motion returning gt; constant
+                                maps resampled to c*sqrt(N/n), DC sample and
+                                mask crops
 ==============================  ===============================================
 """
 from __future__ import annotations

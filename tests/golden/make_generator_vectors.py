@@ -8,6 +8,10 @@ module sources by name (ast) and run with NumPy alone:
   simulation.py:104-174  generate_motion_parameters_new
   simulation.py:24-63    sample_disorder
   utils.py:5-36          generate_motion_parameters
+and writes disorder_orders.npz: orders of the reference's DISORDER sampler
+(simulation.py:192-245, DisorderConfig and make_disorder_order, run with the
+standard library's random, math.ceil and dataclasses) for the tile and
+acceleration settings the reconstruction workloads use.
 Only the outputs are stored.  Run from the repository root:
   python tests/golden/make_generator_vectors.py [reference dir]
 """
@@ -20,13 +24,13 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def load_functions(path, names):
+def load_functions(path, names, extra=None):
     tree = ast.parse(open(path).read(), filename=path)
-    keep = [n for n in tree.body if isinstance(n, ast.FunctionDef) and n.name in names]
+    keep = [n for n in tree.body if isinstance(n, (ast.FunctionDef, ast.ClassDef)) and n.name in names]
     missing = set(names) - {n.name for n in keep}
     if missing:
         raise SystemExit(f"{path}: functions not found: {sorted(missing)}")
-    ns = {"np": np}
+    ns = {"np": np, **(extra or {})}
     exec(compile(ast.Module(body=keep, type_ignores=[]), path, "exec"), ns)
     return ns
 
@@ -63,6 +67,43 @@ def main(ref):
         out[f"dis_{i}_x"] = x_order
     path = os.path.join(HERE, "motion_generators.npz")
     np.savez(path, **out)
+    print(f"wrote {path}: {len(out)} arrays")
+    disorder_vectors(ref)
+
+
+# (K0, K1, tile0, tile1, accel, seed): the reference builds DisorderConfig from
+# the image's (partition, line) extents, tile = (block_par, block_lin) and
+# pat_lines_to_measure = K1 // accel (simulation.py:385-392)
+DISORDER_CASES = [
+    (64, 64, 4, 4, 1, 123456),      # 64^3, 16 shots
+    (128, 128, 8, 8, 2, 123456),    # 128x128x96, 32 shots
+    (192, 192, 8, 12, 2, 123456),   # 192x192x128, 48 shots
+    (256, 256, 16, 16, 2, 123456),  # 256x256x192, 64 shots
+    (96, 96, 12, 8, 2, 123456),     # a non-square tile of the batched sizes
+    (48, 64, 4, 8, 2, 7),           # non-square plane and tile
+    (30, 50, 4, 8, 3, 2**40 + 5),   # extents padded to tile multiples, 41-bit seed
+]
+
+
+def disorder_vectors(ref):
+    import math
+    import random
+    import typing
+    from dataclasses import dataclass
+
+    sim = load_functions(os.path.join(ref, "simulation.py"), ["DisorderConfig", "make_disorder_order"],
+                         extra={"random": random, "ceil": math.ceil, "dataclass": dataclass, "List": typing.List,
+                                "Tuple": typing.Tuple})
+    out = {}
+    for i, (k0, k1, u0, u1, r, sd) in enumerate(DISORDER_CASES):
+        cfg = sim["DisorderConfig"](max_line_number=k1 - 1, max_partition_number=k0 - 1,
+                                    pat_lines_to_measure=k1 // r, block_lin=u1, block_par=u0, seed=sd)
+        order, (L, P) = sim["make_disorder_order"](cfg)
+        out[f"case_{i}_args"] = np.array([k0, k1, u0, u1, r, sd], dtype=np.int64)
+        out[f"case_{i}_order"] = np.array(order, dtype=np.int16)
+        out[f"case_{i}_LP"] = np.array([L, P], dtype=np.int32)
+    path = os.path.join(HERE, "disorder_orders.npz")
+    np.savez_compressed(path, **out)
     print(f"wrote {path}: {len(out)} arrays")
 
 
