@@ -282,6 +282,197 @@ def plain_transform(torch, x, th, kg, rkg, adjoint=False):
     return x if adjoint else translate(x)
 
 
+def plain_deriv(torch, x, th, kg, rkg, p_idx):
+    """RigidTransformDerivativeCuda.apply (transform.py:379-504) written
+    plainly with torch.fft: rotations about z, y, x (the reverse of apply),
+    the three-term product rule over the shears of the differentiated
+    rotation (transform.py:433-474), then the translation, or for p < 3 its
+    derivative -i k_p U (transform.py:422-431)."""
+    q1, q2, q3, t1, t2, t3 = [float(v) for v in th]
+    c64 = torch.complex64
+
+    def fft(a, ax):
+        return torch.fft.fft(a, dim=ax, norm="ortho")
+
+    def ifft(a, ax):
+        return torch.fft.ifft(a, dim=ax, norm="ortho")
+
+    def phases(theta, rkt, rks):
+        return (torch.exp((1j * math.tan(theta / 2) * rkt).to(c64)),
+                torch.exp((-1j * math.sin(theta) * rks).to(c64)))
+
+    def rot(a, theta, rkt, rks, at, as_):
+        vt, vs = phases(theta, rkt, rks)
+        a = ifft(fft(a, at) * vt, at)
+        a = ifft(fft(a, as_) * vs, as_)
+        return ifft(fft(a, at) * vt, at)
+
+    def rot_d(a, theta, rkt, rks, at, as_):
+        vt, vs = phases(theta, rkt, rks)
+        vt_d = (1j * ((1 + math.tan(theta / 2) ** 2) / 2) * rkt * vt).to(c64)
+        vs_d = (-1j * math.cos(theta) * rks * vs).to(c64)
+        a = fft(a, at)
+        o = fft(ifft(fft(ifft(a * vt_d, at), as_) * vs, as_), at) * vt
+        s2 = fft(ifft(fft(ifft(a * vt, at), as_) * vs_d, as_), at) * vt
+        s3 = fft(ifft(fft(ifft(a * vt, at), as_) * vs, as_), at) * vt_d
+        return ifft(o + s2 + s3, at)
+
+    def trans(a, p=None):
+        U = torch.exp(-1j * (q1 * kg["x"] + q2 * kg["y"] + q3 * kg["z"]))
+        if p is not None:
+            U = -1j * kg["xyz"[p]] * U
+        return torch.fft.ifftn(torch.fft.fftn(a, norm="ortho") * U.to(c64), norm="ortho")
+
+    Z = (t3, rkg["tan"]["z"], rkg["sin"]["z"], 0, 1)
+    Y = (t2, rkg["tan"]["y"], rkg["sin"]["y"], 2, 0)
+    X = (t1, rkg["tan"]["x"], rkg["sin"]["x"], 1, 2)
+    if p_idx < 3:
+        return trans(rot(rot(rot(x, *Z), *Y), *X), p_idx)
+    if p_idx == 3:
+        return trans(rot_d(rot(rot(x, *Z), *Y), *X))
+    if p_idx == 4:
+        return trans(rot(rot_d(rot(x, *Z), *Y), *X))
+    return trans(rot(rot(rot_d(x, *Z), *Y), *X))
+
+
+class PlainJointRecon:
+    """JointRecon (joint_recon.py:38-115) over estimate_image_cg
+    (image_estimation.py:74-165) and estimate_transform
+    (transform_estimation.py:36-152), restated plainly with torch.fft on the
+    GPU as the reference's CuPy path runs: dense boolean masks, coil-batched
+    3-D FFTs of every coil, a dense (6, C, N0, N1, N2) Jacobian buffer, 27 dot
+    products per shot (FP32, as cupy.vdot on complex64), the 6x6 solve in FP64
+    on the host, a host synchronisation per dot as the reference's .item() /
+    scalar reads.  Inputs centred, as the reference's JointRecon takes them."""
+
+    def __init__(self, torch, ksp, mps, shot_mask, kgrid, rkgrid, t0=None, max_cg_iter=3, tol=1e-6):
+        import numpy as np
+
+        self.torch = torch
+        dims = (-3, -2, -1)
+        dev = ksp.device
+        self.kg, self.rkg = kgrid, rkgrid
+        self.max_cg_iter, self.tol = max_cg_iter, tol
+        self.ksp = torch.fft.ifftshift(ksp.to(torch.complex64), dim=dims)
+        self.mps = torch.fft.ifftshift(mps.to(torch.complex64), dim=dims)
+        self.mask = torch.fft.ifftshift(torch.as_tensor(np.asarray(shot_mask), device=dev), dim=dims)
+        self.max_norm = float(torch.fft.ifftn(self.ksp, dim=dims, norm="ortho").abs().max())
+        self.ksp /= self.max_norm
+        self.S = self.mask.shape[0]
+        self.x = torch.zeros(self.ksp.shape[1:], dtype=torch.complex64, device=dev)
+        self.t = np.zeros((self.S, 6)) if t0 is None else np.array(t0, dtype=np.float64)
+        self.damp = np.ones(self.S)
+        self.flags = np.zeros(self.S, dtype=bool)
+        self.partial = torch.empty((6, *self.ksp.shape), dtype=torch.complex64, device=dev)
+        ssq = (self.mps.abs() ** 2).sum(0)
+        self.P = 1 / (ssq + 1e-3)
+        self.M = ssq > 0.1
+        self.accepted = []
+
+    def _T(self, v, th, adjoint=False):
+        return plain_transform(self.torch, v, th, self.kg, self.rkg, adjoint=adjoint)
+
+    def _fftn(self, a):
+        return self.torch.fft.fftn(a, dim=(-3, -2, -1), norm="ortho")
+
+    def _ifftn(self, a):
+        return self.torch.fft.ifftn(a, dim=(-3, -2, -1), norm="ortho")
+
+    def _vdot(self, a, b):
+        return float(self.torch.vdot(a.reshape(-1), b.reshape(-1)).real)
+
+    def normal(self, v):
+        """A_func, image_estimation.py:106-129."""
+        out = self.torch.zeros_like(v)
+        for i in range(self.S):
+            Ex = self._ifftn(self._fftn(self.mps * self._T(v, self.t[i])) * self.mask[i])
+            out += self._T((self.mps.conj() * Ex).sum(0), self.t[i], adjoint=True)
+        return out * self.M
+
+    def cg(self):
+        """estimate_image_cg, image_estimation.py:92-165."""
+        b = self.torch.zeros_like(self.x)
+        for i in range(self.S):
+            tmp = self._ifftn(self.mask[i] * self.ksp)
+            b += self._T((self.mps.conj() * tmp).sum(0), self.t[i], adjoint=True)
+            b *= self.M
+        x = self.x.clone()
+        r = b - self.normal(x)
+        z = self.P * r
+        p = z.clone()
+        rsold = self._vdot(r, z)
+        for _ in range(self.max_cg_iter):
+            Ap = self.normal(p)
+            alpha = rsold / self._vdot(p, Ap)
+            x = x + alpha * p
+            r = r - alpha * Ap
+            z = self.P * r
+            rsnew = self._vdot(r, z)
+            if math.sqrt(abs(rsnew)) < self.tol:
+                break
+            p = z + (rsnew / rsold) * p
+            rsold = rsnew
+        self.x = x
+
+    def _residual(self, th):
+        return (self._fftn(self.mps * self._T(self.x, th)) - self.ksp) * self.mask_s
+
+    def lm(self):
+        """estimate_transform, transform_estimation.py:39-152 (use_ga=False)."""
+        import numpy as np
+
+        acc = []
+        for s in range(self.S):
+            self.mask_s = self.mask[s]
+            res = self._residual(self.t[s])
+            e0 = self._vdot(res, res)
+            g = np.zeros(6)
+            for p in range(6):
+                self.partial[p] = self._fftn(self.mps * plain_deriv(self.torch, self.x, self.t[s], self.kg, self.rkg,
+                                                                    p)) * self.mask_s
+                g[p] = self._vdot(res, self.partial[p])
+            H = np.zeros((6, 6))
+            for i in range(6):
+                for j in range(i, 6):
+                    v = self._vdot(self.partial[i], self.partial[j])
+                    H[i, j] = v + self.damp[s] if i == j else v
+                    H[j, i] = H[i, j]
+            nxt = self.t[s] - np.linalg.solve(H, g)
+            res = self._residual(nxt)
+            e1 = self._vdot(res, res)
+            if e1 < e0:
+                if np.all(np.abs(self.t[s] - nxt) < self.tol):
+                    self.flags[s] = True
+                self.t[s] = nxt
+                self.damp[s] = max(self.damp[s] / 5, 1e-4)
+            else:
+                self.damp[s] = min(self.damp[s] * 1.5, 1e16)
+            acc.append(bool(e1 < e0))
+        self.accepted.append(acc)
+        mean = self.t.mean(0)
+        self.x = self._T(self.x, mean)
+        self.t -= mean
+
+    def objective(self):
+        """joint_recon.py:108-115."""
+        total = 0.0
+        for s in range(self.S):
+            self.mask_s = self.mask[s]
+            r = self._residual(self.t[s])
+            total += self._vdot(r, r)
+        return total
+
+    def step(self):
+        """One outer iteration, joint_recon.py:93-102."""
+        self.cg()
+        self.lm()
+        self.x = self.x * self.M
+        return self.objective()
+
+    def image(self):
+        return self.torch.fft.fftshift(self.x * self.max_norm)
+
+
 def plain_bench(args, torch, me, dev):
     """Library against a plain torch.fft restatement of the reference's
     per-shot loop (image_estimation.py:106-129: transform, coil maps, 3-D FFT

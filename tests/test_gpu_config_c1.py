@@ -62,3 +62,24 @@ def test_c1_cg_first_step(me, c1):
     got = me.estimate_image_cg(*args, d["kg"], d["rkg"], d["P"], d["M"], max_iter=1, tol=0.0)
     ref = O.estimate_image_cg(*args, d["okg"], d["orkg"], d["P"], d["M"], max_iter=1, tol=0.0)
     assert _rel(got, ref) < TOL
+
+
+def test_c1_twenty_cg_steps_match_plain_restatement(me, c1):
+    """The configuration's full work, 20 CG-SENSE steps with the known motion,
+    against the plain torch.fft restatement of estimate_image_cg on the same
+    GPU (bench.PlainJointRecon.cg; held to the oracle in
+    tests/test_plain_reference.py): relative L2 1e-4."""
+    import bench
+
+    d = c1
+    dev = torch.device("cuda:0")
+    got = me.estimate_image_cg(d["ksp_u"], d["mps_u"], d["A_u"], d["t"], d["kg"], d["rkg"], d["P"], d["M"],
+                               max_iter=20, tol=0.0)
+    tkg, trkg = me.compute_transform_grids_voxel(d["shape"], (1, 1, 1), xp=torch)
+    ax = (-3, -2, -1)
+    pj = bench.PlainJointRecon(torch, torch.as_tensor(np.fft.fftshift(d["ksp_u"], axes=ax), device=dev),
+                               torch.as_tensor(np.fft.fftshift(d["mps_u"], axes=ax), device=dev),
+                               np.fft.fftshift(d["A_u"], axes=(1, 2)), tkg, trkg, t0=d["t"], max_cg_iter=20, tol=0.0)
+    pj.cg()
+    # PlainJointRecon normalises the data by max |ifftn(ksp)| as JointRecon does; CG is linear in the data
+    assert _rel(got, (pj.x * pj.max_norm).cpu().numpy()) < TOL
