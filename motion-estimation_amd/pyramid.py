@@ -74,11 +74,20 @@ def resize_mask(A, oshape):
 
 
 def pyramid_reconstruction(ksp, mps, A, nshots, n_spatial_levels=3, n_temporal_levels=3, n_joint_iters=100,
-                           save_path=None, sampling="disorder", final_cg_iter=100, compute_objective=True):
+                           save_path=None, sampling="disorder", final_cg_iter=100, compute_objective=True,
+                           final_joint_iters=0):
     """pyramid.py:24-118.  Returns (x centred numpy image, t_est numpy
     (nshots, 6)).  ``n_temporal_levels`` is unused, as in the reference.
     Convergence histories are written to ``save_path`` as
-    loss_<f>_factor.npy when it is given (the reference plots them)."""
+    loss_<f>_factor.npy when it is given (the reference plots them).
+
+    ``final_joint_iters`` > 0 replaces the reference's image-only final level
+    (pyramid.py:96-115) by a JointRecon of that many outer iterations at full
+    resolution starting from the coarse levels' motion (an extension: the
+    reference never estimates motion at full resolution); its image and
+    motion are returned.  ``pyramid_reconstruction.levels`` records each
+    level's factor, shape, objective history, transform history and wall
+    time."""
     dev = _lib.require_gpu()
     ksp = as_c64(ksp, dev)
     mps = as_c64(mps, dev)
@@ -87,7 +96,10 @@ def pyramid_reconstruction(ksp, mps, A, nshots, n_spatial_levels=3, n_temporal_l
     ncoils = ksp.shape[0]
     t_est = np.zeros((nshots, 6), dtype=float)
     levels = []
+    import time
+
     for sl in range(n_spatial_levels - 1):
+        t_level = time.perf_counter()
         f = 2 ** (n_spatial_levels - sl - 1)
         if sampling == "disorder":
             ds = (int(full[0] // f), int(full[1] // f), full[2])
@@ -103,19 +115,33 @@ def pyramid_reconstruction(ksp, mps, A, nshots, n_spatial_levels=3, n_temporal_l
                          compute_objective=compute_objective)
         _, t = app.run()
         t_est = t.detach().cpu().numpy().copy()
-        levels.append(dict(factor=f, shape=ds, objective=app.objective_history, transforms=app.transform_history))
+        levels.append(dict(factor=f, shape=ds, objective=app.objective_history, transforms=app.transform_history,
+                           seconds=time.perf_counter() - t_level))
         if save_path is not None:
             os.makedirs(save_path, exist_ok=True)
             np.save(os.path.join(str(save_path), f"loss_{f}_factor.npy"), app.objective_history)
         del app, mps_res, ksp_res
-    # full resolution: image estimation only (pyramid.py:96-115)
+    t_level = time.perf_counter()
     kgrid, rkgrid = compute_transform_grids_voxel(full, [1, 1, 1], None)
+    if final_joint_iters and final_joint_iters > 0:
+        app = JointRecon(ksp, mps, A, kgrid, rkgrid, t0=t_est, max_joint_iter=int(final_joint_iters),
+                         compute_objective=compute_objective)
+        img, t = app.run()
+        t_est = t.detach().cpu().numpy().copy()
+        torch.cuda.synchronize(dev)
+        levels.append(dict(factor=1, shape=full, objective=app.objective_history, transforms=app.transform_history,
+                           seconds=time.perf_counter() - t_level))
+        pyramid_reconstruction.levels = levels
+        return img.cpu().numpy(), t_est
+    # full resolution: image estimation only (pyramid.py:96-115)
     prob = SenseProblem(ksp, mps, A, kgrid, rkgrid, centered=True, normalize=True)
     prob.theta.copy_(torch.as_tensor(t_est, device=dev))
     x = prob.zeros_image()
     b = prob.rhs()
     prob.cg(x, b, int(final_cg_iter), 1e-6, check_every=1)
     img = prob.image_from_plane(x, shift="f", scale=prob.max_norm)
+    torch.cuda.synchronize(dev)
+    levels.append(dict(factor=1, shape=full, objective=None, transforms=None, seconds=time.perf_counter() - t_level))
     pyramid_reconstruction.levels = levels
     return img.cpu().numpy(), t_est
 
