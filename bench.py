@@ -730,7 +730,7 @@ def full_bench(args, torch, me, dev):
     t0 = time.perf_counter()
     x, t_est = me.pyramid_reconstruction(data["ksp"], data["mps"], data["A"], nshots, n_spatial_levels=3,
                                          n_joint_iters=iters, final_joint_iters=args.final_iters,
-                                         compute_objective=False)
+                                         compute_objective=False, progress=log)
     total_s = time.perf_counter() - t0
     levels = me.pyramid.pyramid_reconstruction.levels
     log(f"pyramid done in {total_s:.1f}s: " + ", ".join(f"x{l['factor']} {l['seconds']:.1f}s" for l in levels))

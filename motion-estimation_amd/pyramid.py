@@ -75,7 +75,7 @@ def resize_mask(A, oshape):
 
 def pyramid_reconstruction(ksp, mps, A, nshots, n_spatial_levels=3, n_temporal_levels=3, n_joint_iters=100,
                            save_path=None, sampling="disorder", final_cg_iter=100, compute_objective=True,
-                           final_joint_iters=0):
+                           final_joint_iters=0, progress=None):
     """pyramid.py:24-118.  Returns (x centred numpy image, t_est numpy
     (nshots, 6)).  ``n_temporal_levels`` is unused, as in the reference.
     Convergence histories are written to ``save_path`` as
@@ -87,7 +87,8 @@ def pyramid_reconstruction(ksp, mps, A, nshots, n_spatial_levels=3, n_temporal_l
     reference never estimates motion at full resolution); its image and
     motion are returned.  ``pyramid_reconstruction.levels`` records each
     level's factor, shape, objective history, transform history and wall
-    time."""
+    time; ``progress`` (a callable taking a message) is told when each level
+    ends."""
     dev = _lib.require_gpu()
     ksp = as_c64(ksp, dev)
     mps = as_c64(mps, dev)
@@ -117,6 +118,8 @@ def pyramid_reconstruction(ksp, mps, A, nshots, n_spatial_levels=3, n_temporal_l
         t_est = t.detach().cpu().numpy().copy()
         levels.append(dict(factor=f, shape=ds, objective=app.objective_history, transforms=app.transform_history,
                            seconds=time.perf_counter() - t_level))
+        if progress is not None:
+            progress(f"level x{f} {ds}: {int(n_joint_iters * f)} outer iterations in {levels[-1]['seconds']:.1f}s")
         if save_path is not None:
             os.makedirs(save_path, exist_ok=True)
             np.save(os.path.join(str(save_path), f"loss_{f}_factor.npy"), app.objective_history)
