@@ -192,7 +192,15 @@ class SenseProblem:
             Mp = torch.empty(vol, dtype=torch.uint8, device=dev)
             _lib.call("me_coil_ssq", _lib.ptr(self.maps), self.ncoils, vol, 1e-3, 0.1, _lib.ptr(Pp), _lib.ptr(Mp), s)
         self.P = real_to_plane(P, self.shape, torch.float32, dev) if P is not None else Pp
-        self.M = real_to_plane(M, self.shape, torch.uint8, dev) if M is not None else Mp
+        if M is not None:
+            Mt = real_to_plane(M, self.shape, torch.float32, dev)
+            # the kernels apply M as a 0/1 support mask (the reference multiplies
+            # by M, image_estimation.py:103, 129; its M is boolean, joint_recon.py:85)
+            if not bool(((Mt == 0) | (Mt == 1)).all()):
+                raise ValueError("M must be a binary (0/1 or boolean) support mask")
+            self.M = Mt.to(torch.uint8)
+        else:
+            self.M = Mp
         # batching over shots
         budget = mem_budget or DEFAULT_MEM_BUDGET
         ncg = int(_lib.size("me_encode_coil_groups", self.ncoils))

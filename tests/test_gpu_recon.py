@@ -314,3 +314,16 @@ def test_masks_along_the_readout_axis(me, prob_data):
     rep[0, 0, 0, 1] = not rep[0, 0, 0, 0]
     with pytest.raises(ValueError, match="readout axis"):
         me.encoding.ShotPlan(rep, torch.device("cuda:0"))
+
+
+def test_fractional_support_mask_is_refused(me, prob_data):
+    """The kernels apply M as a 0/1 support mask; the reference's M is the
+    boolean sum|S|^2 > 0.1 (joint_recon.py:85), so a fractional M is an
+    error rather than being silently truncated."""
+    d = prob_data
+    args = (d["ksp_u"], d["mps_u"], d["A_u"], d["t_true"], d["kg"], d["rkg"], d["P"])
+    with pytest.raises(ValueError):
+        me.estimate_image_cg(*args, 0.5 * d["M"].astype(np.float32), max_iter=1)
+    got = me.estimate_image_cg(*args, d["M"].astype(np.float32), max_iter=1)  # 0/1 floats are fine
+    ref = me.estimate_image_cg(*args, d["M"], max_iter=1)
+    assert torch.equal(got, ref)
