@@ -34,6 +34,8 @@ _SIGS = {
     "me_encode_forward": [vp, i64, vp, i32, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp, i32, vp],
     "me_encode_forward_static": [vp, i64, vp, i32, i32, vp, vp, vp, vp, vp, i32, vp, vp, vp, i32, vp],
     "me_encode_adjoint": [vp, vp, i32, i32, vp, vp, vp, vp, vp, vp, i64, vp],
+    "me_encode_tuning_set": [vp, i32, i32, i32],
+    "me_tune_encode_forward": [vp, i64, vp, i32, i32, vp, vp, vp, vp, vp, i32, vp, i32, vp, vp, vp],
     "me_permute_shift": [vp, vp, vp, vp, vp, vp, vp, C.c_float, vp],
     "me_coil_ssq": [vp, i32, i64, C.c_float, C.c_float, vp, vp, vp],
     "me_gather_hybrid": [vp, i32, vp, vp, vp, vp, vp, i32, i64, C.c_float, vp, vp],
@@ -89,6 +91,7 @@ _SIZES = {
     "me_coil_gram_partials_size": ([i32], C.c_longlong),
     "me_espirit_hankel_size": ([vp, vp], C.c_longlong),
     "me_line_tuning_get": ([vp, vp, i32], C.c_int),
+    "me_encode_tuning_get": ([vp, i32, i32], C.c_int),
     "me_metric_partials_size": ([], C.c_int),
     "me_jacobi_workspace_size": ([i32], C.c_longlong),
     "me_jacobi_set_lds_rows": ([i32], C.c_int),
@@ -96,7 +99,7 @@ _SIZES = {
 }
 _OPTIONAL = {}
 # entry points returning nothing
-_VOID = {"me_line_tuning_clear": []}
+_VOID = {"me_line_tuning_clear": [], "me_encode_tuning_clear": []}
 
 
 def declared_symbols():

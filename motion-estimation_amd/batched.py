@@ -14,9 +14,10 @@ volumes is recorded once into a HIP graph and replayed:
   that the GPU runs concurrently, filling the device when single volumes are
   too small to do so.
 
-``tune()`` runs the line-pass autotuner on each distinct volume geometry
-before capture, so the recorded kernels use the fastest lines-per-workgroup
-choice.  Replays compute exactly what the eager calls compute: the kernels
+``tune()`` runs the autotuners on each distinct volume geometry before
+capture (the line passes' lines per workgroup, the forward encoding's coils
+per workgroup and samples per thread), so the recorded kernels use the
+fastest choices.  Replays compute exactly what the eager calls compute: the kernels
 and their per-volume order are the same.  No counterpart in the reference,
 which reconstructs one volume per process.
 """
@@ -43,6 +44,28 @@ def tune_problem(pb, reps=5):
     return tuple(chosen), tuple(round(t, 4) for t in times)
 
 
+def tune_encoders(pb, reps=3):
+    """Pick the forward-encoding variant (coils per workgroup, samples per
+    thread) for each shot batch of one problem by timing all of them on its
+    buffers; returns {max_cnt: (chosen variant, mean ms per variant)}.
+    The samples do not depend on the choice."""
+    out = {}
+    pl = pb.plan
+    U, Y = pb._ubuf(), pb._ybuf()
+    for s0, s1 in pb.batches():
+        maxc = int(pl.cnt_h[s0:s1].max()) if s1 > s0 else 0
+        if maxc in out:
+            continue
+        chosen = C.c_int()
+        times = (C.c_float * 3)()
+        _lib.call("me_tune_encode_forward", _lib.ptr(U), pb.vol, _lib.ptr(pb.maps), pb.ncoils, s1 - s0,
+                  _lib.int3(pb.shape), _lib.ptr(pl.sk0), _lib.ptr(pl.sk1), _lib.ptr(pl.off[s0:s1]),
+                  _lib.ptr(pl.cnt[s0:s1]), maxc, _lib.ptr(Y), int(reps), C.byref(chosen), times,
+                  _lib.stream(pb.device))
+        out[maxc] = (int(chosen.value), tuple(round(t, 4) for t in times))
+    return out
+
+
 class BatchedJointRecon:
     def __init__(self, volumes, max_cg_iter=3, max_nm_iter=1, nstreams=4):
         """volumes: iterable of dicts with keys ksp, mps, shot_mask, kgrid,
@@ -60,17 +83,26 @@ class BatchedJointRecon:
         self._capture_stream = None
         self.steps_done = 0
         self.tuning = {}
+        self.encoder_tuning = {}
 
     # ------------------------------------------------------------ tuning
     def tune(self, reps=5):
-        """Autotune the plain line passes of every distinct volume geometry
-        (plane-major layout, the problem's shot batch) on its scratch buffer.
-        Returns {shape: (chosen lines per axis, per-candidate ms)}."""
-        done = {}
+        """Autotune every distinct volume geometry before capture: the plain
+        line passes' lines per workgroup (plane-major layout, the problem's
+        shot batch) and the forward encoding's variant (coils per workgroup,
+        samples per thread) for each (shape, coils, largest shot).  Returns
+        {shape: (chosen lines per axis, per-candidate ms)} and keeps the
+        encoder picks in ``encoder_tuning``."""
+        done, enc = {}, {}
         for r in self.recons:
-            if r.problem.shape not in done:
-                done[r.problem.shape] = tune_problem(r.problem, reps)
+            pb = r.problem
+            if pb.shape not in done:
+                done[pb.shape] = tune_problem(pb, reps)
+            key = (pb.shape, pb.ncoils, pb.plan.max_cnt)
+            if key not in enc:
+                enc[key] = tune_encoders(pb, max(1, reps // 2))
         self.tuning = done
+        self.encoder_tuning = enc
         return done
 
     # ------------------------------------------------------------ stepping

@@ -22,8 +22,11 @@
 //
 // Volumes (image u, maps S_c) are in plane-major layout: element (x0, x1, x2)
 // at (x2*N0 + x0)*N1 + x1.  Samples of a shot are sorted by (k1, k0).
+#include <algorithm>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 
 #include "me_device.h"
 #include "me_internal.h"
@@ -31,7 +34,6 @@
 
 namespace me {
 
-constexpr int kCoilsPerWG = 16;
 constexpr int kMaxCol = 512;  // maximum N0 (column length, LDS twiddle table)
 constexpr int kAdjChunk = 512;  // samples per coil staged in LDS by the adjoint
 
@@ -39,8 +41,24 @@ template <int NR>
 struct RowT {
   static constexpr int R1 = LineFactor<NR>::R1;
   static constexpr int R2 = LineFactor<NR>::R2;
-  static constexpr int T = R1 > R2 ? R1 : R2;
-  static constexpr int NT = T * kCoilsPerWG;
+  static constexpr int T = R1 > R2 ? R1 : R2;  // threads per row
+};
+
+// Forward-encoding variants: G coils per workgroup (one row of T threads per
+// coil) and SPT samples accumulated per thread.  A shot whose samples exceed
+// SPT * T * G is split into chunks that each stream all rows again.  Every
+// variant computes each sample with the same operations (bitwise the same
+// samples); only the grouping of the FP64 norm partials differs.
+struct FwdVariant {
+  int G, SPT;
+};
+constexpr FwdVariant kFwdVariants[] = {{16, 2}, {8, 2}, {8, 4}};
+constexpr int kNumFwdVariants = 3;
+
+template <int NR, int G>
+struct FwdT {
+  static constexpr int R1 = RowT<NR>::R1, R2 = RowT<NR>::R2, T = RowT<NR>::T;
+  static constexpr int NT = T * G;
 };
 
 struct EncArgs {
@@ -83,10 +101,9 @@ __device__ __forceinline__ double block_sum_f64(double v, double* sh) {
   return s;
 }
 
-template <int NR, int SPT>
-__global__ void __launch_bounds__(RowT<NR>::NT) k_encode_fwd(EncArgs A) {
-  constexpr int R1 = RowT<NR>::R1, R2 = RowT<NR>::R2, T = RowT<NR>::T, NT = RowT<NR>::NT;
-  constexpr int G = kCoilsPerWG;
+template <int NR, int SPT, int G>
+__global__ void __launch_bounds__((FwdT<NR, G>::NT)) k_encode_fwd(EncArgs A) {
+  constexpr int R1 = FwdT<NR, G>::R1, R2 = FwdT<NR, G>::R2, T = FwdT<NR, G>::T, NT = FwdT<NR, G>::NT;
   // exchange rows padded to XS (even, so each row starts 16-byte aligned and
   // the step-2 reads go in pairs)
   constexpr int XS = R2 + 2;
@@ -752,16 +769,24 @@ __global__ void k_maxabs(const float2* __restrict__ x, long long n, unsigned int
 }
 
 namespace {
-template <int SPT>
+template <int SPT, int G>
 struct FwdLaunch {
   const EncArgs* a;
   dim3 grid;
   hipStream_t s;
   template <int NR>
   void operator()() {
-    hipLaunchKernelGGL((k_encode_fwd<NR, SPT>), grid, dim3(RowT<NR>::NT), 0, s, *a);
+    hipLaunchKernelGGL((k_encode_fwd<NR, SPT, G>), grid, dim3(FwdT<NR, G>::NT), 0, s, *a);
   }
 };
+bool launch_fwd(int variant, int n1, const EncArgs* a, dim3 grid, hipStream_t s) {
+  switch (variant) {
+    case 0: { FwdLaunch<2, 16> f{a, grid, s}; return dispatch_line_length(n1, f); }
+    case 1: { FwdLaunch<2, 8> f{a, grid, s}; return dispatch_line_length(n1, f); }
+    case 2: { FwdLaunch<4, 8> f{a, grid, s}; return dispatch_line_length(n1, f); }
+    default: return false;
+  }
+}
 // lines of at most 16 threads use the wave-per-phase adjoint
 template <int NR>
 constexpr bool adj_wave() { return RowT<NR>::T <= 16; }
@@ -792,15 +817,37 @@ struct AdjRows {
   void operator()() { *rows = adj_wave<NR>() ? AdjW<NR>::L : AdjT<NR>::L; }
 };
 struct RowThreads {
-  int* nt;
+  int* t;
   template <int NR>
-  void operator()() { *nt = RowT<NR>::NT; }
+  void operator()() { *t = RowT<NR>::T; }
 };
+// threads per row of N1 (0: unsupported length)
 int threads_for_row(int n1) {
-  int nt = 0;
-  RowThreads f{&nt};
+  int t = 0;
+  RowThreads f{&t};
   dispatch_line_length(n1, f);
-  return nt;
+  return t;
+}
+// workgroups per shot of a forward-encoding variant
+struct FwdGeom {
+  int ncg, nchunk, blocks_per_shot;
+};
+FwdGeom fwd_geom(int variant, int n1, int n2, int ncoils, int max_cnt) {
+  const FwdVariant v = kFwdVariants[variant];
+  const int nt = threads_for_row(n1) * v.G;
+  FwdGeom g{};
+  g.ncg = (ncoils + v.G - 1) / v.G;
+  g.nchunk = max_cnt > 0 ? (max_cnt + v.SPT * nt - 1) / (v.SPT * nt) : 1;
+  g.blocks_per_shot = n2 * g.ncg * g.nchunk;
+  return g;
+}
+// tuned variant per (N0, N1, N2, coils, largest shot)
+std::mutex g_enc_mu;
+std::map<std::tuple<int, int, int, int, int>, int> g_enc_tune;
+int fwd_variant(const int* n, int ncoils, int max_cnt) {
+  std::lock_guard<std::mutex> lk(g_enc_mu);
+  const auto it = g_enc_tune.find(std::make_tuple(n[0], n[1], n[2], ncoils, max_cnt));
+  return it == g_enc_tune.end() ? 0 : it->second;
 }
 }  // namespace
 
@@ -848,8 +895,9 @@ int encode_forward_impl(const void* u, long long u_bstride, const void* maps, in
   ME_REQUIRE(nbatch >= 1 && nbatch <= 65535 && ncoils >= 1, "encode_forward: bad batch/coil count");
   ME_REQUIRE(n[0] <= kMaxCol, "encode_forward: N0 above 512");
   ME_REQUIRE(!norms || partials, "encode_forward: norms need a partials workspace");
-  const int nt = threads_for_row(n[1]);
-  ME_REQUIRE(nt > 0, "encode_forward: unsupported N1 " + std::to_string(n[1]));
+  ME_REQUIRE(threads_for_row(n[1]) > 0, "encode_forward: unsupported N1 " + std::to_string(n[1]));
+  const int variant = fwd_variant(n, ncoils, max_cnt);
+  const FwdGeom fg = fwd_geom(variant, n[1], n[2], ncoils, max_cnt);
   EncArgs A{};
   A.u = (const float2*)u;
   A.u_bs = u_bstride;
@@ -866,15 +914,13 @@ int encode_forward_impl(const void* u, long long u_bstride, const void* maps, in
   A.meas = (const float2*)y_meas;
   A.partial = partials;
   A.store_raw = (flags & 1) ? 1 : 0;
-  A.ncg = (ncoils + kCoilsPerWG - 1) / kCoilsPerWG;
-  const int spt = 2;
-  A.nchunk = max_cnt > 0 ? (max_cnt + spt * nt - 1) / (spt * nt) : 1;
-  A.blocks_per_shot = n[2] * A.ncg * A.nchunk;
+  A.ncg = fg.ncg;
+  A.nchunk = fg.nchunk;
+  A.blocks_per_shot = fg.blocks_per_shot;
   A.nbatch = nbatch;
   ME_REQUIRE((long long)A.blocks_per_shot * nbatch < (1LL << 31), "encode_forward: grid too large");
   dim3 grid((unsigned)(A.blocks_per_shot * nbatch), 1u);
-  FwdLaunch<2> f{&A, grid, stream};
-  ME_REQUIRE(dispatch_line_length(n[1], f), "encode_forward: unsupported N1");
+  ME_REQUIRE(launch_fwd(variant, n[1], &A, grid, stream), "encode_forward: unsupported N1");
   ME_TRY_HIP(hipGetLastError());
   if (norms) {
     hipLaunchKernelGGL(k_sum_partials, dim3(nbatch), dim3(256), 0, stream, partials, A.blocks_per_shot, norms);
@@ -890,12 +936,69 @@ using namespace me;
 extern "C" {
 
 long long me_encode_partials_size(int n2, int ncoils, int max_cnt, int n1) {
-  const int nt = threads_for_row(n1);
-  if (nt == 0 || ncoils <= 0 || n2 <= 0) return 0;
-  const int ncg = (ncoils + kCoilsPerWG - 1) / kCoilsPerWG;
-  const int spt = 2;
-  const int nchunk = max_cnt > 0 ? (max_cnt + spt * nt - 1) / (spt * nt) : 1;
-  return (long long)n2 * ncg * nchunk;
+  if (threads_for_row(n1) == 0 || ncoils <= 0 || n2 <= 0) return 0;
+  long long most = 0;  // enough for whichever variant is tuned in
+  for (int v = 0; v < kNumFwdVariants; ++v) most = std::max(most, (long long)fwd_geom(v, n1, n2, ncoils, max_cnt).blocks_per_shot);
+  return most;
+}
+
+int me_encode_tuning_set(const int* n, int ncoils, int max_cnt, int variant) {
+  ME_REQUIRE(n && ncoils >= 1, "encode_tuning_set: bad argument");
+  ME_REQUIRE(variant >= 0 && variant < kNumFwdVariants, "encode_tuning_set: variant must be 0, 1 or 2");
+  std::lock_guard<std::mutex> lk(g_enc_mu);
+  const auto key = std::make_tuple(n[0], n[1], n[2], ncoils, max_cnt);
+  if (variant == 0)
+    g_enc_tune.erase(key);
+  else
+    g_enc_tune[key] = variant;
+  return 0;
+}
+
+int me_encode_tuning_get(const int* n, int ncoils, int max_cnt) { return n ? fwd_variant(n, ncoils, max_cnt) : -1; }
+
+void me_encode_tuning_clear(void) {
+  std::lock_guard<std::mutex> lk(g_enc_mu);
+  g_enc_tune.clear();
+}
+
+int me_tune_encode_forward(const void* u, long long u_bstride, const void* maps, int ncoils, int nbatch, const int* n,
+                           const int* samp_k0, const int* samp_k1, const int* shot_off, const int* shot_cnt,
+                           int max_cnt, void* y, int reps, int* chosen, float* times_ms, hipStream_t stream) {
+  ME_REQUIRE(u && maps && n && y && reps >= 1, "tune_encode_forward: bad argument");
+  float best = -1.f;
+  int pick = 0;
+  hipEvent_t e0, e1;
+  ME_TRY_HIP(hipEventCreate(&e0));
+  ME_TRY_HIP(hipEventCreate(&e1));
+  int rc = 0;
+  for (int v = 0; v < kNumFwdVariants && rc == 0; ++v) {
+    rc = me_encode_tuning_set(n, ncoils, max_cnt, v);
+    for (int r = 0; r <= reps && rc == 0; ++r) {  // the first launch is untimed
+      if (r == 1 && hipEventRecord(e0, stream) != hipSuccess) rc = 1;
+      if (rc == 0)
+        rc = encode_forward_impl(u, u_bstride, maps, ncoils, nbatch, n, samp_k0, samp_k1, shot_off, shot_cnt,
+                                 max_cnt, y, nullptr, nullptr, nullptr, 0, stream);
+    }
+    float ms = 0.f;
+    if (rc == 0 && (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = 1;
+    ms /= (float)reps;
+    if (times_ms) times_ms[v] = ms;
+    if (rc == 0 && (best < 0.f || ms < best)) {
+      best = ms;
+      pick = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != 0) {
+    me_encode_tuning_set(n, ncoils, max_cnt, 0);
+    if (rc == 1) ::me::set_error("tune_encode_forward: HIP event timing failed");
+    return rc;
+  }
+  if (chosen) *chosen = pick;
+  return me_encode_tuning_set(n, ncoils, max_cnt, pick);
 }
 
 int me_encode_forward(const void* u, long long u_bstride, const void* maps, int ncoils, int nbatch, const int* n,
@@ -912,7 +1015,9 @@ int me_encode_forward_static(const void* u, long long u_bstride, const void* map
   double* partials = nullptr;
   if (norms) {
     ME_REQUIRE(n && nbatch >= 1, "encode_forward_static: bad argument");
-    const long long need = me_encode_partials_size(n[2], ncoils, max_cnt, n[1]) * (long long)nbatch;
+    ME_REQUIRE(threads_for_row(n[1]) > 0, "encode_forward_static: unsupported N1 " + std::to_string(n[1]));
+    const long long need =
+        (long long)fwd_geom(fwd_variant(n, ncoils, max_cnt), n[1], n[2], ncoils, max_cnt).blocks_per_shot * nbatch;
     ME_REQUIRE(need <= kStaticPartials,
                "encode_forward_static: " + std::to_string(need) + " FP64 norm partials needed, the static scratch "
                "holds " + std::to_string(kStaticPartials) + " (ME_ENCODE_STATIC_PARTIALS); use me_encode_forward "

@@ -65,3 +65,19 @@ def test_grid_builder_matches_oracle(me):
         assert shp == tuple(kg.shape)
         for a, b in zip(rv, kg.r):
             assert np.allclose(a, b, atol=1e-12)
+
+
+def test_encoder_tuning_table_without_device(me):
+    """The forward-encoding variant table is host state: set, get, refuse an
+    unknown variant, clear."""
+    lib = me._lib.load()
+    n = me._lib.int3((64, 64, 48))
+    assert lib.me_encode_tuning_get(n, 9, 256) == 0
+    assert lib.me_encode_tuning_set(n, 9, 256, 2) == 0
+    assert lib.me_encode_tuning_get(n, 9, 256) == 2
+    assert lib.me_encode_tuning_get(n, 10, 256) == 0
+    assert lib.me_encode_tuning_set(n, 9, 256, 3) != 0 and b"variant" in lib.me_last_error()
+    lib.me_encode_tuning_clear()
+    assert lib.me_encode_tuning_get(n, 9, 256) == 0
+    # the partials workspace covers every variant: 8-coil groups need more at 9 coils
+    assert lib.me_encode_partials_size(48, 9, 256, 64) >= 48 * 2

@@ -91,17 +91,22 @@ def test_batched_graph_matches_one_at_a_time(me):
              dict(shape=(32, 32, 24), coils=8, tile=(4, 8), accel=1, nshots=32, seed=14)]
     vols = _volumes(me, specs)
     n_iter = 4
-    ref = []
-    for v in vols:
-        jr = me.JointRecon(v["ksp"], v["mps"], v["shot_mask"], v["kgrid"], v["rkgrid"], max_joint_iter=n_iter,
-                           compute_objective=False)
-        for _ in range(n_iter):
-            jr.step()
-        ref.append((jr.image().cpu(), jr.t.cpu()))
     bj = me.BatchedJointRecon(vols, nstreams=3)
     try:
+        # tune first: the process-wide tables then serve the eager reference too
         picks = bj.tune(reps=2)
         assert set(picks) == {sp["shape"] for sp in specs}
+        assert len(bj.encoder_tuning) == len({(sp["shape"], sp["coils"]) for sp in specs})
+        for enc in bj.encoder_tuning.values():
+            for variant, ms in enc.values():
+                assert variant in (0, 1, 2) and min(ms) > 0
+        ref = []
+        for v in vols:
+            jr = me.JointRecon(v["ksp"], v["mps"], v["shot_mask"], v["kgrid"], v["rkgrid"], max_joint_iter=n_iter,
+                               compute_objective=False)
+            for _ in range(n_iter):
+                jr.step()
+            ref.append((jr.image().cpu(), jr.t.cpu()))
         imgs, ts = bj.run(n_iter)
         torch.cuda.synchronize()
         assert bj.steps_done == n_iter and bj._exec is not None
@@ -112,3 +117,4 @@ def test_batched_graph_matches_one_at_a_time(me):
     finally:
         bj.close()
         me._lib.load().me_line_tuning_clear()
+        me._lib.load().me_encode_tuning_clear()

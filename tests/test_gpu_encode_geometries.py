@@ -87,3 +87,53 @@ def test_normal_operator_and_rhs(case):
     refb = O.cg_rhs(d["ksp"], d["mps_u"], d["A_u"], d["t"], d["okg"], d["orkg"], d["M"])
     assert _rel(pb.image_from_plane(pb.rhs(cache=False)), refb) < TOL
     assert _rel(pb.image_from_plane(pb.rhs(cache=True)), refb) < TOL
+
+
+def test_forward_variants_bitwise_equal_and_at_parity(me, case):
+    """Every forward-encoding variant (16 coils x 2 samples per thread, 8 x 2,
+    8 x 4; me_encode_tuning_set) gives bitwise the same samples, residual
+    norms equal to FP64 rounding, and the oracle's encoding."""
+    d = case
+    pb, shape, nc = d["pb"], d["shape"], d["nc"]
+    L = me._lib
+    x = pb.image_to_plane(d["rng"].standard_normal(shape) + 1j * d["rng"].standard_normal(shape))
+    key = (L.int3(shape), nc, pb.plan.max_cnt)
+    outs = []
+    try:
+        for v in (0, 1, 2):
+            L.call("me_encode_tuning_set", *key, v)
+            assert L.size("me_encode_tuning_get", *key) == v
+            Y = pb.forward(x).clone()
+            norms = pb.objective_per_shot(x)
+            outs.append((Y, norms))
+    finally:
+        L.load().me_encode_tuning_clear()
+    assert L.size("me_encode_tuning_get", *key) == 0
+    for Y, norms in outs[1:]:
+        assert torch.equal(Y, outs[0][0])
+        assert float((norms - outs[0][1]).abs().max() / outs[0][1].abs().max()) < 1e-12
+    pl = pb.plan
+    n2 = shape[2]
+    xh = pb.image_from_plane(x).cpu().numpy()
+    Y = outs[2][0].cpu().numpy()
+    dense = O.encode_forward(xh, d["mps_u"], d["A_u"][0], d["t"][0], d["okg"], d["orkg"])
+    hyb = np.fft.ifft(dense, axis=-1, norm="ortho")
+    c = pl.cnt_h[0]
+    ref = hyb[:, pl.k0_h[:c], pl.k1_h[:c], :]
+    assert _rel(np.transpose(Y[:c * n2 * nc].reshape(n2, nc, c), (1, 2, 0)), ref) < TOL
+
+
+def test_forward_encoder_tuner_records_its_pick(me, case):
+    import ctypes as C
+
+    d = case
+    pb, shape, nc = d["pb"], d["shape"], d["nc"]
+    L = me._lib
+    try:
+        picks = me.batched.tune_encoders(pb, reps=2)
+        (variant, ms), = picks.values()
+        assert variant in (0, 1, 2) and min(ms) > 0
+        assert ms[variant] == min(ms)
+        assert L.size("me_encode_tuning_get", L.int3(shape), nc, pb.plan.max_cnt) == variant
+    finally:
+        L.load().me_encode_tuning_clear()
