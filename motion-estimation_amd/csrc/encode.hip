@@ -474,18 +474,49 @@ __global__ void __launch_bounds__(AdjT<NR>::NT) k_encode_adj_rows(EncArgs A) {
 // are lane-local.  The exchange then hands each row to T threads (the
 // line-pass layout, consecutive x1 across lanes) for the length-R2 DFTs, the
 // conj(S_c) product and the coil sum in registers.
+//
+// Row twiddles e^{-2 pi i t k1/NR} (t < R2 the column phase, k1 < R1),
+// phase-major: a wavefront's R1 twiddles are one contiguous run of 2 R1
+// floats, fetched as a few wide scalar loads under one wait (indexing the
+// length-NR table by (t k1) mod NR costs one dependent scalar load each).
+template <int NR, int R1, int R2>
+struct PhaseTw {
+  alignas(64) float v[2 * R1 * R2];
+  constexpr PhaseTw() : v{} {
+    for (int t = 0; t < R2; ++t)
+      for (int k = 0; k < R1; ++k) {
+        const CS w = ce_cs_2pi((long)t * k, NR);
+        v[2 * (t * R1 + k)] = (float)w.c;
+        v[2 * (t * R1 + k) + 1] = -(float)w.s;
+      }
+  }
+};
+template <int NR, int R1, int R2>
+__device__ constexpr PhaseTw<NR, R1, R2> kPhTw{};
+
 template <int NR>
 struct AdjW {
   static constexpr int R1 = RowT<NR>::R1, R2 = RowT<NR>::R2, T = RowT<NR>::T;
   static constexpr int L = 64;
   static constexpr int NT = T * L;
   static constexpr int RS = R1 * (R2 + 1) + 1;  // exchange row stride (odd)
-  static constexpr int OFF_REC = kMaxCol * 8;   // column twiddles at LDS offset 0
+  static constexpr int TWB = 8192;               // column twiddle tables at LDS offset 0
+  static constexpr int OFF_REC = TWB;
   static constexpr int OFF_XCH = OFF_REC + kAdjChunk * 16;
   static constexpr int OFF_CP = OFF_XCH + L * RS * 8;
   static constexpr int BYTES = OFF_CP + (NR + 1) * 4;
+  static_assert(kMaxCol * 8 <= TWB, "one twiddle table of the longest column");
 };
 
+// The scatter's cost is the latency of its dependent chain (record read,
+// twiddle read, complex FMA) per sample.  For a power-of-two N0 the twiddle
+// address is (k0 x0 mod N0): with k0 = 2^a k' the 64 rows of a wavefront
+// touch only N0/2^a entries spaced 2^a apart, a 2^a-way bank conflict.  So the
+// twiddle area holds one table per 2-adic order a (entry j of table a is
+// e^{-2 pi i (j 2^a mod N0)/N0}) and a record carries k' and the byte base of
+// table a: consecutive rows then read consecutive-stride entries of an odd
+// multiplier, spread over all banks.  Orders beyond the last table use it with
+// an even k'.
 template <int NR, bool POW2>
 __global__ void __launch_bounds__(AdjW<NR>::NT) k_encode_adj_wave(EncArgs A) {
   using AW = AdjW<NR>;
@@ -515,11 +546,19 @@ __global__ void __launch_bounds__(AdjW<NR>::NT) k_encode_adj_wave(EncArgs A) {
   const int mask8 = 8 * n0 - 1;
   const float inv_n0 = 1.0f / (float)n0;
 
-  for (int j = tid; j < n0; j += NT) {
+  int nsub = 1;  // twiddle tables: 2-adic orders 0 .. nsub-1 (POW2), one table otherwise
+  if (POW2) {
+    int lg = 0;
+    while ((1 << lg) < n0) ++lg;
+    nsub = min(AW::TWB / (8 * n0), lg + 1);
+  }
+  for (int e = tid; e < nsub * n0; e += NT) {
+    const int a = e / n0, j = e % n0;
+    const int jm = (int)(((long long)j << a) % n0);
     float s, co;
-    const int jj = (2 * j > n0) ? j - n0 : j;
+    const int jj = (2 * jm > n0) ? jm - n0 : jm;
     sincospif(-2.0f * (float)jj / (float)n0, &s, &co);
-    twc[j] = make_float2(co, s);
+    twc[e] = make_float2(co, s);
   }
   for (int j = tid; j <= n1; j += NT) cp[j] = A.colptr[(long long)b * (n1 + 1) + j];
 
@@ -539,7 +578,13 @@ __global__ void __launch_bounds__(AdjW<NR>::NT) k_encode_adj_wave(EncArgs A) {
     __syncthreads();  // readers of the previous chunk's tables are done
     for (int j = tid; j < ns; j += NT) {
       const int k0 = A.sk0[off + s0 + j];
-      rec[j].z = __int_as_float(POW2 ? 8 * k0 : k0);
+      if (POW2) {
+        const int a = k0 ? min(__builtin_ctz(k0), nsub - 1) : 0;
+        rec[j].z = __int_as_float(8 * (k0 >> a));
+        rec[j].w = __int_as_float(a * 8 * n0);  // byte base of table a
+      } else {
+        rec[j].z = __int_as_float(k0);
+      }
     }
     float2 ypre[SPT];
 #pragma unroll
@@ -585,10 +630,17 @@ __global__ void __launch_bounds__(AdjW<NR>::NT) k_encode_adj_wave(EncArgs A) {
           const float4 q = *(const float4*)(recb + o);
           const int k = __float_as_int(q.z);
           float2 w;
-          if (POW2)
-            w = *(const float2*)(smem + (int)(__umul24(k, xr) & mask8));
-          else
+          if (POW2) {
+            // table base | 8 ((k' x0) mod n0), the base a multiple of 8 n0, as one
+            // v_and_or_b32 (the compiler emits a separate and and or here)
+            int ad;
+            asm("v_and_or_b32 %0, %1, %2, %3"
+                : "=v"(ad)
+                : "v"((int)__umul24(k, xr)), "s"(mask8), "v"(__float_as_int(q.w)));
+            w = *(const float2*)(smem + ad);
+          } else {
             w = twc[mulmod(k, xr, n0, -1, inv_n0)];
+          }
           z = c_fmac(w, make_float2(q.x, q.y), z);  // twc holds e^{-2 pi i j/n0}: conjugate
         };
         // one accumulator per column, zeroed once for all columns up front
@@ -607,13 +659,11 @@ __global__ void __launch_bounds__(AdjW<NR>::NT) k_encode_adj_wave(EncArgs A) {
           if (o < oe) term(a[m1], o);
         }
         RDFT<R1, +1>::run(a);
-        {  // row twiddles from the constant table: t is the wavefront's column phase
+        {  // row twiddles of the wavefront's column phase t
           const int tu = __builtin_amdgcn_readfirstlane(t);
+          const float* tp = kPhTw<NR, R1, R2>.v + 2 * R1 * tu;
 #pragma unroll
-          for (int k1 = 1; k1 < R1; ++k1) {
-            const int j = (tu * k1) % NR;
-            a[k1] = c_mulc(a[k1], make_float2(kTw<NR>.c[j], -kTw<NR>.s[j]));
-          }
+          for (int k1 = 1; k1 < R1; ++k1) a[k1] = c_mulc(a[k1], make_float2(tp[2 * k1], tp[2 * k1 + 1]));
         }
 #pragma unroll
         for (int k1 = 0; k1 < R1; ++k1) xch[r * RS + k1 * (R2 + 1) + t] = a[k1];
