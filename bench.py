@@ -37,7 +37,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c5", help="c1, c2, c3 or c5 (default: the headline workload)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the CPU-baseline leg: the oracle timing and the small-size FP64-oracle accuracy check")
     ap.add_argument("--no-objective", action="store_true", help="skip the per-iteration objective")
     ap.add_argument("--no-tune", action="store_true", help="skip the line-pass autotuner at setup")
     ap.add_argument("--grad", action="store_true",
@@ -256,6 +257,40 @@ def cpu_baseline(shape, ncoils, nshots, n_cg):
                 unit_seconds={k: round(v, 2) for k, v in t.items()})
 
 
+def small_problem(me, shape, nc, seed, rot_deg=3.0, trans_vox=1.5):
+    """A seeded motion-corrupted problem small enough for the FP64 oracle
+    (centred frame, numpy): phantom, coil maps, DISORDER shots (tile 4 x 4,
+    acceleration 2), uniform random motion, k-space corrupted by the oracle's
+    generate_corrupted_kspace plus noise at 1e-3 of the peak."""
+    import numpy as np
+
+    from oracle import motion_oracle as O
+
+    sim = me.simulation
+    gt = sim.phantom(shape, seed=seed).numpy().astype(np.complex128)
+    mps = sim.coil_maps(shape, nc, seed=seed + 1).numpy().astype(np.complex128)
+    A, _ = sim.disorder_sampling(shape, (4, 4), 2)
+    t_true = sim.motion_uniform(A.shape[0], rot_deg=rot_deg, trans_vox=trans_vox, seed=seed + 2)
+    ksp = O.generate_corrupted_kspace(gt, mps, A, t_true)
+    rng = np.random.default_rng(seed + 3)
+    ksp = ksp + 1e-3 * np.abs(ksp).max() * (rng.standard_normal(ksp.shape) + 1j * rng.standard_normal(ksp.shape))
+    return gt, mps, A, ksp
+
+
+def _motion_err(t, t_ref):
+    import numpy as np
+
+    d = np.abs(np.asarray(t) - np.asarray(t_ref))
+    return float(max(d[:, :3].max(), np.degrees(d[:, 3:]).max()))
+
+
+def _rel(a, b):
+    import numpy as np
+
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
 def oracle_accuracy(me, dev):
     """Error of the measured path against the FP64 oracle on the same seeded
     inputs at a size the oracle finishes in about a second: three outer
@@ -266,15 +301,8 @@ def oracle_accuracy(me, dev):
 
     from oracle import motion_oracle as O
 
-    shape, nc = (32, 24, 16), 4
-    sim = me.simulation
-    gt = sim.phantom(shape, seed=0).numpy().astype(np.complex128)
-    mps = sim.coil_maps(shape, nc, seed=1).numpy().astype(np.complex128)
-    A, _ = sim.disorder_sampling(shape, (4, 4), 2)
-    t_true = sim.motion_uniform(A.shape[0], rot_deg=3.0, trans_vox=1.5, seed=2)
-    ksp = O.generate_corrupted_kspace(gt, mps, A, t_true)
-    rng = np.random.default_rng(3)
-    ksp = ksp + 1e-3 * np.abs(ksp).max() * (rng.standard_normal(ksp.shape) + 1j * rng.standard_normal(ksp.shape))
+    shape = (32, 24, 16)
+    gt, mps, A, ksp = small_problem(me, shape, 4, 0)
     kg, rkg = me.compute_transform_grids_voxel(shape, (1, 1, 1))
     okg, orkg = O.compute_transform_grids_voxel(shape, (1, 1, 1))
     app = me.JointRecon(ksp, mps, A, kg, rkg, max_cg_iter=3, max_joint_iter=3)
@@ -290,6 +318,177 @@ def oracle_accuracy(me, dev):
                                            / ref.objective_history)),
                 damping_identical=bool(np.array_equal(app.damp.cpu().numpy(), ref.damp)),
                 bars=dict(image_rel_l2=1e-4, motion=1e-3, objective_rel=1e-4))
+
+
+def grad_accuracy(me, dev):
+    """The data-consistency loss and both gradients (image, motion) of the
+    measured path against the FP64 oracle's dc_loss / dc_loss_grad at
+    16 x 12 x 8, 4 coils, 24 DISORDER shots, random data and motion."""
+    import numpy as np
+    import torch
+
+    from oracle import motion_oracle as O
+
+    shape, nc = (16, 12, 8), 4
+    sim = me.simulation
+    rng = np.random.default_rng(40)
+    mps = np.fft.ifftshift(sim.coil_maps(shape, nc, seed=41).numpy().astype(np.complex128), axes=(-3, -2, -1))
+    A, _ = sim.disorder_sampling(shape, (4, 4), 2)
+    A = np.fft.ifftshift(A, axes=(1, 2))
+    ksp = rng.standard_normal((nc, *shape)) + 1j * rng.standard_normal((nc, *shape))
+    t = sim.motion_uniform(A.shape[0], rot_deg=4.0, trans_vox=1.5, seed=42)
+    x0 = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    kg, rkg = me.compute_transform_grids_voxel(shape, (1, 1, 1))
+    okg, orkg = O.compute_transform_grids_voxel(shape, (1, 1, 1))
+    pb = me.encoding.SenseProblem(ksp, mps, A, kg, rkg)
+    x = torch.as_tensor(x0.astype(np.complex64), device=dev).requires_grad_(True)
+    th = torch.as_tensor(t, device=dev).requires_grad_(True)
+    loss = me.data_consistency_loss(x, th, pb)
+    loss.backward()
+    ref = O.dc_loss(x0, t, ksp, mps, A, okg, orkg)
+    gx, gt = O.dc_loss_grad(x0, t, ksp, mps, A, okg, orkg)
+    return dict(workload=f"data-consistency loss and gradients, 16x12x8, 4 coils, {A.shape[0]} shots",
+                loss_rel=float(abs(float(loss.detach()) - ref) / ref),
+                grad_image_rel_l2=_rel(x.grad.cpu().numpy(), gx), grad_motion_rel_l2=_rel(th.grad.cpu().numpy(), gt),
+                bars=dict(loss_rel=1e-4, grad_image_rel_l2=1e-4, grad_motion_rel_l2=1e-4))
+
+
+def normal_accuracy(me, dev):
+    """The forward encoding E_s T_s of every shot and the normal operator
+    sum_s T_s^H E_s^H E_s T_s of the measured path against the FP64 oracle's
+    encode_forward / normal_op at 32 x 24 x 16, 4 coils, 8 DISORDER shots,
+    random image and motion (uncentred frame, as the reference's operators)."""
+    import numpy as np
+    import torch
+
+    from oracle import motion_oracle as O
+
+    shape, nc = (32, 24, 16), 4
+    sim = me.simulation
+    mps = np.fft.ifftshift(sim.coil_maps(shape, nc, seed=51).numpy().astype(np.complex128), axes=(-3, -2, -1))
+    A, _ = sim.disorder_sampling(shape, (4, 4), 2)
+    A = np.fft.ifftshift(A, axes=(1, 2))
+    t = sim.motion_uniform(A.shape[0], rot_deg=3.0, trans_vox=1.5, seed=52)
+    rng = np.random.default_rng(53)
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    kg, rkg = me.compute_transform_grids_voxel(shape, (1, 1, 1))
+    okg, orkg = O.compute_transform_grids_voxel(shape, (1, 1, 1))
+    pb = me.encoding.SenseProblem(np.zeros((nc, *shape), np.complex64), mps, A, kg, rkg)
+    pb.theta.copy_(torch.as_tensor(t))
+    Y = pb.forward(pb.image_to_plane(x)).cpu().numpy()
+    pl, n2 = pb.plan, shape[2]
+    fwd = []
+    for s in range(A.shape[0]):
+        hyb = np.fft.ifft(O.encode_forward(x, mps, A[s], t[s], okg, orkg), axis=-1, norm="ortho")
+        o, c = int(pl.off_h[s]), int(pl.cnt_h[s])
+        ref = hyb[:, pl.k0_h[o:o + c], pl.k1_h[o:o + c], :]
+        got = np.transpose(Y[o * n2 * nc:(o + c) * n2 * nc].reshape(n2, nc, c), (1, 2, 0))
+        fwd.append(_rel(got, ref))
+    out = pb.zeros_image()
+    pb.normal(pb.image_to_plane(x), out)
+    M = np.sum(np.abs(mps) ** 2, axis=0) > 0.1
+    ref = O.normal_op(x, mps, A, t, okg, orkg, M)
+    return dict(workload=f"E_s T_s per shot and the normal operator, 32x24x16, 4 coils, {A.shape[0]} shots",
+                forward_rel_l2_max_over_shots=float(max(fwd)),
+                normal_rel_l2=_rel(pb.image_from_plane(out).cpu().numpy(), ref),
+                bars=dict(forward_rel_l2_max_over_shots=1e-4, normal_rel_l2=1e-4))
+
+
+def batch_accuracy(me, dev):
+    """Two volumes of different size batched (graph replay over 2 streams,
+    three outer iterations of 3 CG + LM) against the FP64 oracle's JointRecon
+    of each volume alone."""
+    from oracle import motion_oracle as O
+
+    probs = [((32, 24, 16), 4, 60), ((24, 24, 12), 3, 70)]
+    vols, data = [], []
+    for shape, nc, seed in probs:
+        _, mps, A, ksp = small_problem(me, shape, nc, seed)
+        kg, rkg = me.compute_transform_grids_voxel(shape, (1, 1, 1))
+        vols.append(dict(ksp=ksp, mps=mps, shot_mask=A, kgrid=kg, rkgrid=rkg))
+        data.append((shape, mps, A, ksp))
+    bj = me.BatchedJointRecon(vols, nstreams=2)
+    try:
+        imgs, ts = bj.run(3)
+        got = [(i.cpu().numpy(), t.cpu().numpy()) for i, t in zip(imgs, ts)]
+    finally:
+        bj.close()
+    img_err, mot_err = [], []
+    for (shape, mps, A, ksp), (xg, tg) in zip(data, got):
+        okg, orkg = O.compute_transform_grids_voxel(shape, (1, 1, 1))
+        xr, tr = O.JointRecon(ksp, mps, A, okg, orkg, max_cg_iter=3, max_joint_iter=3, compute_objective=False).run()
+        img_err.append(_rel(xg, xr))
+        mot_err.append(_motion_err(tg, tr))
+    return dict(workload="BatchedJointRecon graph replay, 3 outer iterations of 3 CG + LM, volumes 32x24x16 "
+                         f"(4 coils, {data[0][2].shape[0]} shots) and 24x24x12 (3 coils, {data[1][2].shape[0]} shots), "
+                         "each against the oracle alone",
+                image_rel_l2_max=float(max(img_err)), motion_max_abs_deg_vox=float(max(mot_err)),
+                bars=dict(image_rel_l2_max=1e-4, motion_max_abs_deg_vox=1e-3))
+
+
+def pyramid_accuracy(me, dev):
+    """The pyramid driver (one coarse JointRecon level, then CG-only at full
+    resolution) against the FP64 oracle's pyramid_reconstruction at
+    32 x 32 x 8, 4 coils, DISORDER shots."""
+    from oracle import motion_oracle as O
+
+    _, mps, A, ksp = small_problem(me, (32, 32, 8), 4, 80, rot_deg=2.0, trans_vox=1.0)
+    kw = dict(n_spatial_levels=2, n_joint_iters=1, final_cg_iter=20)
+    x, t = me.pyramid_reconstruction(ksp, mps, A, A.shape[0], compute_objective=False, **kw)
+    xr, tr = O.pyramid_reconstruction(ksp, mps, A, A.shape[0], **kw)
+    return dict(workload=f"pyramid, 2 levels (x2 JointRecon 2 outer, x1 CG 20), 32x32x8, 4 coils, {A.shape[0]} shots",
+                image_rel_l2=_rel(x, xr), motion_max_abs_deg_vox=_motion_err(t, tr),
+                bars=dict(image_rel_l2=1e-4, motion_max_abs_deg_vox=1e-3))
+
+
+def calib_accuracy(me, dev):
+    """The three calibration stages against the FP64 oracle at small sizes:
+    noise whitening (12 coils, 16 x 12 x 8), coil compression (16 coils,
+    24 x 20 x 12, perc 0.95: singular values and the compression subspace)
+    and ESPIRiT (4 coils, 24 x 20 x 16, calibration 12 x 12 x 10, kernel 3^3:
+    kept vectors, singular values and the maps where both are non-zero)."""
+    import numpy as np
+
+    from oracle import motion_oracle as O
+
+    sim = me.simulation
+    rng = np.random.default_rng(90)
+    C = 12
+    mix = rng.standard_normal((C, C)) + 1j * rng.standard_normal((C, C))
+    noise = (mix @ (rng.standard_normal((C, 5000)) + 1j * rng.standard_normal((C, 5000)))).astype(np.complex64)
+    ksp = (rng.standard_normal((C, 16, 12, 8)) + 1j * rng.standard_normal((C, 16, 12, 8))).astype(np.complex64)
+    wg, _ = me.noise_whiten(ksp, noise)
+    wr, _ = O.noise_whiten(ksp, noise)
+    shape = (24, 20, 12)
+    mps = sim.coil_maps(shape, 16, seed=91).numpy()
+    kc = (rng.standard_normal((16, *shape)) + 1j * rng.standard_normal((16, *shape))).astype(np.complex64)
+    me.coil_compression(mps, kc, perc=0.95)
+    _, _, Ar, S = O.coil_compression(mps, kc, perc=0.95)
+    Ag = me.coil_compression.matrix.cpu().numpy().astype(np.complex128)
+    lam = me.coil_compression.singular_values
+    imsize, nc = (24, 20, 16), 4
+    m4 = sim.coil_maps(imsize, nc, seed=92).numpy().astype(np.complex128)
+    img = m4 * sim.phantom(imsize, seed=93).numpy()[None]
+    k4 = np.fft.fftshift(np.fft.fftn(np.fft.ifftshift(img, axes=(1, 2, 3)), axes=(1, 2, 3), norm="ortho"), axes=(1, 2, 3))
+    sl = tuple(slice(n // 2 - c // 2, n // 2 - c // 2 + c) for n, c in zip(imsize, (12, 12, 10)))
+    calib = np.ascontiguousarray(np.moveaxis(k4[(slice(None),) + sl], 0, -1)).astype(np.complex64)
+    info = {}
+    er = O.rx_espirit_3d(calib, imsize, kernel_size=(3, 3, 3), info=info)
+    eg = me.rx_espirit_3d(calib, imsize, kernel_size=(3, 3, 3)).cpu().numpy().astype(np.complex128)
+    both = np.any(er != 0, axis=-1) & np.any(eg != 0, axis=-1)
+    Se, se = info["singular_values"], me.rx_espirit_3d.singular_values
+    k = int(np.sum(Se > 1e-3 * Se[0]))
+    return dict(workload="noise whitening 12 coils 16x12x8; coil compression 16 coils 24x20x12 perc 0.95; "
+                         "ESPIRiT 4 coils 24x20x16, calibration 12x12x10, kernel 3^3",
+                whitened_rel_l2=_rel(wg.cpu().numpy(), wr),
+                compression_singular_values_max_err_over_s0=float(np.max(np.abs(lam - S)) / S[0]),
+                compression_subspace_rel=_rel(Ag.conj().T @ Ag, Ar.conj().T @ Ar),
+                espirit_kept_equal=bool(me.rx_espirit_3d.kept == info["kept"]),
+                espirit_singular_values_max_err_over_s0=float(np.max(np.abs(se[:k] - Se[:k])) / Se[0]),
+                espirit_maps_rel_l2_common_support=_rel(eg[both], er[both]),
+                bars=dict(whitened_rel_l2=1e-5, compression_singular_values_max_err_over_s0=1e-6,
+                          compression_subspace_rel=1e-6, espirit_singular_values_max_err_over_s0=1e-6,
+                          espirit_maps_rel_l2_common_support=1e-4))
 
 
 def grad_bench(args, torch, me, dev):
@@ -353,6 +552,9 @@ def grad_bench(args, torch, me, dev):
         "fd_check": {"directions": 64, "step": h, "median_rel": float(np.median(rel)), "max_rel": float(max(rel)),
                      "max_rel_to_gradient_scale": float(max(scaled))},
     }
+    if not args.no_cpu_baseline:
+        log("checking the same path against the FP64 oracle at a small size")
+        out["accuracy_vs_fp64_oracle"] = grad_accuracy(me, dev)
     print(json.dumps(out), flush=True)
 
 
@@ -652,6 +854,9 @@ def plain_bench(args, torch, me, dev):
     torch.cuda.empty_cache()
     out["outer_iteration"] = plain_outer(args, torch, me, dev)
     out["speedup_over_plain"]["outer_iteration"] = out["outer_iteration"]["speedup"]
+    if not args.no_cpu_baseline:
+        log("checking the same path against the FP64 oracle at a small size")
+        out["accuracy_vs_fp64_oracle"] = normal_accuracy(me, dev)
     print(json.dumps(out), flush=True)
 
 
@@ -768,6 +973,9 @@ def full_bench(args, torch, me, dev):
         "image_error_magnitude_rel_l2": {"corrected": err(xg), "uncorrected_sense": err(unc),
                                          "sense_seconds": round(sense_s, 2)},
     }
+    if not args.no_cpu_baseline:
+        log("checking the same path against the FP64 oracle at a small size")
+        out["accuracy_vs_fp64_oracle"] = pyramid_accuracy(me, dev)
     print(json.dumps(out), flush=True)
 
 
@@ -843,6 +1051,9 @@ def batch_bench(args, torch, me, dev):
                                                                          for m, v in e.items()}
                            for k, e in bj.encoder_tuning.items()},
     }
+    if not args.no_cpu_baseline:
+        log("checking the same path against the FP64 oracle at a small size")
+        out["accuracy_vs_fp64_oracle"] = batch_accuracy(me, dev)
     print(json.dumps(out), flush=True)
 
 
@@ -900,7 +1111,8 @@ def calib_bench(args, torch, me, dev):
     }
     if not args.no_cpu_baseline:
         from oracle import motion_oracle as O
-        nthreads = min(16, os.cpu_count() or 1)
+        nthreads, aff = host_threads()
+        O.set_workers(nthreads)
         ksp_h = ksp.cpu().numpy()
         mps_h = maps_c.cpu().numpy()
         del ksp, maps_c
@@ -922,11 +1134,13 @@ def calib_bench(args, torch, me, dev):
         t_e0 = time.perf_counter() - t0
         t_e = t_e0 + (t_e2 - t_e0) / 2 * shape[2]
         out["cpu_baseline"] = {"value": round(t_w + t_c + t_e, 1), "unit": "s", "cores": nthreads, "kind": "port",
+                               "cpu_model": cpu_model(), "affinity_cores": aff,
                                "s": {"noise_whiten": round(t_w, 2), "rx_espirit_3d": round(t_e, 1),
                                      "coil_compression": round(t_c, 2)},
                                "sample": f"FP64 oracle: whitening and compression on 1/{frac} of the voxels, ESPIRiT "
                                          "calibration-matrix SVD in full plus 2 of 192 z-slices of the per-voxel "
                                          "stage, each scaled to the full volume"}
+        out["accuracy_vs_fp64_oracle"] = calib_accuracy(me, dev)
     print(json.dumps(out), flush=True)
 
 
