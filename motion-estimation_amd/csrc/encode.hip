@@ -509,14 +509,14 @@ struct AdjW {
 };
 
 // The scatter's cost is the latency of its dependent chain (record read,
-// twiddle read, complex FMA) per sample.  For a power-of-two N0 the twiddle
-// address is (k0 x0 mod N0): with k0 = 2^a k' the 64 rows of a wavefront
-// touch only N0/2^a entries spaced 2^a apart, a 2^a-way bank conflict.  So the
-// twiddle area holds one table per 2-adic order a (entry j of table a is
-// e^{-2 pi i (j 2^a mod N0)/N0}) and a record carries k' and the byte base of
-// table a: consecutive rows then read consecutive-stride entries of an odd
-// multiplier, spread over all banks.  Orders beyond the last table use it with
-// an even k'.
+// twiddle read, complex FMA) per sample.  The twiddle index is (k0 x0 mod
+// N0): with k0 = 2^a k' the 64 rows of a wavefront touch entries spaced 2^a
+// apart, a bank conflict of that order.  So the twiddle area holds one table
+// per 2-adic order a up to that of N0 (entry j of table a is
+// e^{-2 pi i (j 2^a mod N0)/N0}; (2^a (k' x0 mod N0)) mod N0 = k0 x0 mod N0
+// for any N0) and a record carries k' and the byte base of table a:
+// consecutive rows then read entries of an odd multiplier, spread over the
+// banks.  Orders beyond the last table use it with an even k'.
 template <int NR, bool POW2>
 __global__ void __launch_bounds__(AdjW<NR>::NT) k_encode_adj_wave(EncArgs A) {
   using AW = AdjW<NR>;
@@ -546,12 +546,8 @@ __global__ void __launch_bounds__(AdjW<NR>::NT) k_encode_adj_wave(EncArgs A) {
   const int mask8 = 8 * n0 - 1;
   const float inv_n0 = 1.0f / (float)n0;
 
-  int nsub = 1;  // twiddle tables: 2-adic orders 0 .. nsub-1 (POW2), one table otherwise
-  if (POW2) {
-    int lg = 0;
-    while ((1 << lg) < n0) ++lg;
-    nsub = min(AW::TWB / (8 * n0), lg + 1);
-  }
+  // twiddle tables: 2-adic orders 0 .. nsub-1 (up to the order of n0 itself)
+  const int nsub = min(AW::TWB / (8 * n0), __builtin_ctz(n0) + 1);
   for (int e = tid; e < nsub * n0; e += NT) {
     const int a = e / n0, j = e % n0;
     const int jm = (int)(((long long)j << a) % n0);
@@ -578,13 +574,9 @@ __global__ void __launch_bounds__(AdjW<NR>::NT) k_encode_adj_wave(EncArgs A) {
     __syncthreads();  // readers of the previous chunk's tables are done
     for (int j = tid; j < ns; j += NT) {
       const int k0 = A.sk0[off + s0 + j];
-      if (POW2) {
-        const int a = k0 ? min(__builtin_ctz(k0), nsub - 1) : 0;
-        rec[j].z = __int_as_float(8 * (k0 >> a));
-        rec[j].w = __int_as_float(a * 8 * n0);  // byte base of table a
-      } else {
-        rec[j].z = __int_as_float(k0);
-      }
+      const int a = k0 ? min(__builtin_ctz(k0), nsub - 1) : 0;
+      rec[j].z = __int_as_float(POW2 ? 8 * (k0 >> a) : (k0 >> a));
+      rec[j].w = __int_as_float(a * 8 * n0);  // byte base of table a
     }
     float2 ypre[SPT];
 #pragma unroll
@@ -639,7 +631,7 @@ __global__ void __launch_bounds__(AdjW<NR>::NT) k_encode_adj_wave(EncArgs A) {
                 : "v"((int)__umul24(k, xr)), "s"(mask8), "v"(__float_as_int(q.w)));
             w = *(const float2*)(smem + ad);
           } else {
-            w = twc[mulmod(k, xr, n0, -1, inv_n0)];
+            w = *(const float2*)(smem + 8 * mulmod(k, xr, n0, -1, inv_n0) + __float_as_int(q.w));
           }
           z = c_fmac(w, make_float2(q.x, q.y), z);  // twc holds e^{-2 pi i j/n0}: conjugate
         };
