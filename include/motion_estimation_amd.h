@@ -320,14 +320,20 @@ int me_lm_solve(const double* stats, const double* damp, const double* theta, in
                 double* delta, int nbatch, hipStream_t stream);
 
 /* Geodesic acceleration, the reference's experimental option
- * (transform_estimation.py:73-121).  me_lm_ga_direction: per shot,
- * out = sum_p v_p (Je_p - J_p) / eps over the gathered samples, v = -delta,
- * J the columns at theta and Je those at theta + eps v (same layout as
- * me_lm_gram).  me_lm_ga_solve: with stats_dir = me_lm_gram of out against J,
- * rhs_p = -Re<J'v, J_p>, a = (Gram + damp I)^-1 rhs, and
- * cand = theta + v + a/2 when |a| <= |v|, else theta + v. */
-int me_lm_ga_direction(const void* J, const void* Je, long long jstride, const int* shot_off, const int* shot_cnt,
-                       int per_sample, const double* delta, double eps, void* out, int nbatch, hipStream_t stream);
+ * (transform_estimation.py:73-121).  me_lm_ga_accumulate: per shot b,
+ * out_b = (accumulate ? out_b : 0) + coef * v_b[p_idx] * d_b over n values,
+ * v = -delta; called with the derivative image of column p at theta + h v
+ * (coef = 1/(2h)) and at theta - h v (coef = -1/(2h)) for every p, it builds
+ * the image whose encoding is J'v, the directional derivative of the Jacobian
+ * (transform_estimation.py:78-100 estimates it by a forward difference at
+ * eps = 1e-4 of FP32 columns; the central difference at h = 1e-3 estimates
+ * the same quantity with a tenth of the rounding noise).
+ * me_lm_ga_solve: with stats_dir = me_lm_gram of J'v
+ * against J, rhs_p = -Re<J'v, J_p>, a = (Gram + damp I)^-1 rhs, and
+ * cand = theta + v + a/2 when |a| <= |v|, else theta + v
+ * (transform_estimation.py:102-121). */
+int me_lm_ga_accumulate(const void* d, long long d_bstride, long long n, int nbatch, const double* delta, int p_idx,
+                        double coef, int accumulate, void* out, long long out_bstride, hipStream_t stream);
 int me_lm_ga_solve(const double* stats, const double* stats_dir, const double* damp, const double* theta,
                    int theta_stride, const double* delta, double* cand, int nbatch, hipStream_t stream);
 

@@ -46,7 +46,7 @@ _SIGS = {
     "me_cg_step": [vp, vp, vp, vp, vp, i64, C.c_double, vp, vp, vp, vp, i64, vp],
     "me_csub": [vp, vp, vp, i64, vp],
     "me_dot_re_batched": [vp, i64, vp, i64, i64, i32, vp, vp, i64, vp],
-    "me_lm_ga_direction": [vp, vp, i64, vp, vp, i32, vp, C.c_double, vp, i32, vp],
+    "me_lm_ga_accumulate": [vp, i64, i64, i32, vp, i32, C.c_double, i32, vp, i64, vp],
     "me_lm_ga_solve": [vp, vp, vp, vp, i32, vp, vp, i32, vp],
     "me_sum_batch": [vp, i64, i32, i64, vp, i32, vp],
     "me_axpy_real": [vp, C.c_double, vp, i64, vp],

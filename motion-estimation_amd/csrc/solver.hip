@@ -353,32 +353,26 @@ __global__ void k_lm_solve(const double* __restrict__ stats, const double* __res
   }
 }
 
-// Geodesic acceleration (transform_estimation.py:78-100): per shot,
-// out = sum_p v_p (Je_p - J_p) / eps over the shot's gathered samples, with
-// v = -delta, J the columns at theta and Je those at theta + eps v.
-__global__ void __launch_bounds__(kVecThreads) k_lm_ga_direction(const float2* __restrict__ J,
-                                                                 const float2* __restrict__ Je, long long jstride,
-                                                                 const int* __restrict__ off,
-                                                                 const int* __restrict__ cnt, int per_sample,
-                                                                 const double* __restrict__ delta, float inv_eps,
-                                                                 float2* __restrict__ out) {
+// Geodesic acceleration (transform_estimation.py:78-100), directional
+// derivative J'v of the Jacobian along v = -delta, formed in image space:
+// out_b (+)= coef * v_b[p] * d_b for one derivative image d_b per shot (the
+// column-p image at a perturbed theta), coef = +-1/(2h) for a central
+// difference.  Encoding the accumulated image once gives J'v (the encoding
+// is linear).  Accumulation is FP32.
+__global__ void __launch_bounds__(kVecThreads) k_lm_ga_accumulate(const float2* __restrict__ d, long long d_bs,
+                                                                  long long n, const double* __restrict__ delta,
+                                                                  int p_idx, double coef, int accumulate,
+                                                                  float2* __restrict__ out, long long out_bs) {
   const int b = blockIdx.y;
-  const long long base = (long long)off[b] * per_sample;
-  const long long n = (long long)cnt[b] * per_sample;
-  float v[6];
-#pragma unroll
-  for (int p = 0; p < 6; ++p) v[p] = (float)(-delta[(long long)b * 6 + p]);
+  const float w = (float)(-coef * delta[(long long)b * 6 + p_idx]);
+  const float2* db = d + (long long)b * d_bs;
+  float2* ob = out + (long long)b * out_bs;
   for (long long i = (long long)blockIdx.x * kVecThreads + threadIdx.x; i < n; i += (long long)gridDim.x * kVecThreads) {
-    float2 jev = make_float2(0.f, 0.f), jv = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int p = 0; p < 6; ++p) {
-      const float2 a = Je[p * jstride + base + i], c = J[p * jstride + base + i];
-      jev.x = fmaf(v[p], a.x, jev.x);
-      jev.y = fmaf(v[p], a.y, jev.y);
-      jv.x = fmaf(v[p], c.x, jv.x);
-      jv.y = fmaf(v[p], c.y, jv.y);
-    }
-    out[base + i] = make_float2((jev.x - jv.x) * inv_eps, (jev.y - jv.y) * inv_eps);
+    const float2 a = db[i];
+    float2 o = accumulate ? ob[i] : make_float2(0.f, 0.f);
+    o.x = fmaf(w, a.x, o.x);
+    o.y = fmaf(w, a.y, o.y);
+    ob[i] = o;
   }
 }
 
@@ -555,13 +549,15 @@ int me_lm_solve(const double* stats, const double* damp, const double* theta, in
   return 0;
 }
 
-int me_lm_ga_direction(const void* J, const void* Je, long long jstride, const int* shot_off, const int* shot_cnt,
-                       int per_sample, const double* delta, double eps, void* out, int nbatch, hipStream_t stream) {
-  ME_REQUIRE(J && Je && shot_off && shot_cnt && delta && out && eps > 0, "lm_ga_direction: bad argument");
-  ME_REQUIRE(nbatch >= 1 && nbatch <= 65535, "lm_ga_direction: bad batch count");
-  hipLaunchKernelGGL(k_lm_ga_direction, dim3(64, nbatch), dim3(kVecThreads), 0, stream, (const float2*)J,
-                     (const float2*)Je, jstride, shot_off, shot_cnt, per_sample, delta, (float)(1.0 / eps),
-                     (float2*)out);
+int me_lm_ga_accumulate(const void* d, long long d_bstride, long long n, int nbatch, const double* delta, int p_idx,
+                        double coef, int accumulate, void* out, long long out_bstride, hipStream_t stream) {
+  ME_REQUIRE(d && delta && out && n >= 1, "lm_ga_accumulate: bad argument");
+  ME_REQUIRE(p_idx >= 0 && p_idx < 6, "lm_ga_accumulate: p_idx out of range");
+  ME_REQUIRE(nbatch >= 1 && nbatch <= 65535, "lm_ga_accumulate: bad batch count");
+  const long long blocks = std::min<long long>((n + kVecThreads - 1) / kVecThreads, 1024);
+  hipLaunchKernelGGL(k_lm_ga_accumulate, dim3((unsigned)blocks, nbatch), dim3(kVecThreads), 0, stream,
+                     (const float2*)d, d_bstride, n, delta, p_idx, coef, accumulate ? 1 : 0, (float2*)out,
+                     out_bstride);
   ME_TRY_HIP(hipGetLastError());
   return 0;
 }

@@ -160,12 +160,13 @@ def test_estimate_transform_matches_oracle(me, prob_data):
 
 
 def test_estimate_transform_geodesic_acceleration_matches_oracle(me, prob_data):
-    """use_ga=True (transform_estimation.py:73-121).  J'v is a forward
-    difference at eps = 1e-4 of FP32 Jacobian columns (as in the reference's
-    own FP32 arithmetic), so it carries about 1e-3 relative rounding noise
-    that enters the half-step correction a/2; the motion tolerance for this
-    option is therefore 5e-3 (deg / voxel) and at most a tenth of the
-    acceleration's own effect, instead of the 1e-3 of the plain step."""
+    """use_ga=True (transform_estimation.py:73-121) against the oracle's FP64
+    forward difference at eps = 1e-4.  The library estimates J'v by an FP32
+    central difference at h = 1e-3 (an FP32 forward difference at 1e-4, the
+    reference's own arithmetic, deviates from the FP64 one by about 8e-4 deg
+    here); the plain step's bars hold: motion 1e-3 (deg / voxel), image 1e-4,
+    damping identical, and the acceleration itself must be visible (its
+    effect on the motion is 6e-2 here)."""
     d = prob_data
     x = O.estimate_image_cg(d["ksp_u"], d["mps_u"], d["A_u"], d["t_true"], d["okg"], d["orkg"], d["P"], d["M"],
                             max_iter=5)
@@ -184,8 +185,9 @@ def test_estimate_transform_geodesic_acceleration_matches_oracle(me, prob_data):
     assert np.array_equal(damp_got, damp_ref)
     err = _motion_err(t_got, t_ref)
     print(f"GA: effect {effect:.3e}, GPU vs oracle {err:.3e}")
-    assert err < 5e-3 and err < 0.1 * effect
-    assert _rel(x_got, x_ref) < 1e-3
+    assert effect > 10 * MOTION_TOL
+    assert err < MOTION_TOL
+    assert _rel(x_got, x_ref) < TOL
 
 
 def test_estimate_transform_requires_transforms(me, prob_data):
