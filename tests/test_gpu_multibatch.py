@@ -97,6 +97,25 @@ def test_estimate_transform(me, data):
     assert _rel(x_got, x_ref) < TOL
 
 
+def test_estimate_transform_geodesic_acceleration(me, data):
+    """use_ga=True over three shot batches (the central-difference image is a
+    per-batch buffer): same bars as the plain step."""
+    d = data
+    x = O.estimate_image_cg(d["ksp_u"], d["mps_u"], d["A_u"], d["t"], d["okg"], d["orkg"], d["P"], d["M"],
+                            max_iter=5)
+    t0 = d["t"] + np.random.default_rng(9).normal(0, 0.05, d["t"].shape)
+    ns = t0.shape[0]
+    t_ref, damp_ref = t0.copy(), np.ones(ns)
+    x_ref = O.estimate_transform(d["ksp_u"], d["mps_u"], d["A_u"], x, d["okg"], d["orkg"], damp_ref,
+                                 np.zeros(ns, dtype=bool), t_ref, use_ga=True)
+    t_got, damp_got = t0.copy(), np.ones(ns)
+    x_got = me.estimate_transform(d["ksp_u"], d["mps_u"], d["A_u"], x, d["kg"], d["rkg"], damp_got,
+                                  np.zeros(ns, dtype=bool), None, None, None, None, transforms=t_got, use_ga=True)
+    assert np.array_equal(damp_got, damp_ref)
+    assert _motion_err(t_got, t_ref) < MOTION_TOL
+    assert _rel(x_got, x_ref) < TOL
+
+
 def test_joint_recon(me, data):
     d = data
     kw = dict(max_cg_iter=3, max_joint_iter=3)
