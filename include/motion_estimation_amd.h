@@ -69,12 +69,13 @@ int me_rigid_deriv(const void* x, long long x_bstride, void* y, long long y_bstr
                    double* coef_ws, int p_idx, hipStream_t stream);
 
 /* The six derivative columns of me_rigid_deriv with the rotation chain shared:
- * c1 = Rz x, c2 = Ry c1, c3 = Rx c2 (one volume per batch entry each, at
- * c_bstride) are computed when build != 0 (call p_idx = 0 first with build = 1,
- * then p_idx = 1..5 with build = 0 and the same theta); column p_idx is then
- * produced from the shortest chain level that contains it.  Same result as
- * me_rigid_deriv (transform.py:476-504) with 42 instead of 66 line passes per
- * six columns. */
+ * c1 = Rz x, c2 = Ry c1, c3 = Tr Rx c2 = T_d x (one volume per batch entry
+ * each, at c_bstride) are computed when build != 0 (call p_idx = 0 first with
+ * build = 1, then p_idx = 1..5 with build = 0 and the same theta); column
+ * p_idx is then produced from the shortest chain level that contains it (a
+ * translation column is one -i k_p pass over c3).  The same operator as
+ * me_rigid_deriv (transform.py:476-504), equal to FP32 rounding, with 38
+ * instead of 66 line passes per six columns. */
 int me_rigid_deriv_chain(const void* x, long long x_bstride, void* c1, void* c2, void* c3, long long c_bstride,
                          void* y, long long y_bstride, void* work, long long work_bstride, int nbatch, const int* n,
                          const long long* st, const double* r0, const double* r1, const double* r2,

@@ -227,28 +227,35 @@ std::vector<Pass> build_deriv(int p) {
 
 // Shared-chain derivative columns.  With T_d = Tr Rx Ry Rz, the six columns
 // need the rotation chain up to different levels:
-//   c1 = Rz x,  c2 = Ry c1,  c3 = Rx c2 (no translation folded)
-//   p = 0,1,2: Tr'_p c3            (three translation passes, axis-p one differentiated)
+//   c1 = Rz x,  c2 = Ry c1,  c3 = Tr Rx c2 = T_d x (Tr_y folded into Rx's
+//   last pass, then Tr_x and Tr_z passes)
+//   p = 0,1,2: D_p c3              (one pass along axis p with the factor -i k_p;
+//                                   D_p commutes with the translations)
 //   p = 3:     Tr dRx c2           (Tr_y folded into dRx's last pass, then Tr_x, Tr_z)
 //   p = 4:     Tr Rx dRy c1
 //   p = 5:     Tr Rx Ry dRz x
-// 9 + 3*3 + 5 + 8 + 11 = 42 passes for all six columns instead of 6*11.
+// 11 + 3*1 + 5 + 8 + 11 = 38 passes for all six columns instead of 6*11.
 std::vector<Pass> build_rotation_only(int rho) {
   std::vector<Pass> v;
   add_rotation(v, rho, false, -1, false, false, false);
   return v;
 }
 
+// Last level of the shared chain: c3 = Tr Rx c2 (the whole T_d x).
+std::vector<Pass> build_chain_last() {
+  std::vector<Pass> v;
+  add_rotation(v, 0, false, 1, false, false, false);  // Tr_y folded into the last axis-1 pass
+  v.push_back(single(0, trans(0, false)));
+  v.push_back(single(2, trans(2, false)));
+  return v;
+}
+
 std::vector<Pass> build_deriv_from_chain(int p) {
   std::vector<Pass> v;
-  if (p <= 2) {
-    HostPhase ty = trans(1, false), tx = trans(0, false), tz = trans(2, false);
-    if (p == 0) tx = with_dtrans(tx);
-    if (p == 1) ty = with_dtrans(ty);
-    if (p == 2) tz = with_dtrans(tz);
-    v.push_back(single(1, ty));
-    v.push_back(single(0, tx));
-    v.push_back(single(2, tz));
+  if (p <= 2) {  // -i k_p along axis p of c3 = T_d x
+    HostPhase d;
+    d.active = true;
+    v.push_back(single(p, with_dtrans(d)));
     return v;
   }
   if (p == 5) {
@@ -446,11 +453,11 @@ int me_rigid_deriv_chain(const void* x, long long x_bstride, void* c1, void* c2,
     const void* src[3] = {x, c1, c2};
     const long long sbs[3] = {x_bstride, c_bstride, c_bstride};
     void* dst[3] = {c1, c2, c3};
-    const int rho[3] = {2, 1, 0};
+    const int rho[2] = {2, 1};
     for (int l = 0; l < 3; ++l) {
       Bufs b{(const float2*)src[l], sbs[l], 1, (float2*)dst[l], c_bstride, nullptr, 0};
-      if (int rc = run_passes(build_rotation_only(rho[l]), g, b, nbatch, coef_ws, nullptr, nullptr, false, stream))
-        return rc;
+      const std::vector<Pass> passes = l < 2 ? build_rotation_only(rho[l]) : build_chain_last();
+      if (int rc = run_passes(passes, g, b, nbatch, coef_ws, nullptr, nullptr, false, stream)) return rc;
     }
   }
   const void* from = p_idx <= 2 ? c3 : (p_idx == 3 ? c2 : (p_idx == 4 ? c1 : x));

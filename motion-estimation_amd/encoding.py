@@ -281,8 +281,9 @@ class SenseProblem:
 
     def _deriv_chain(self, x, out, theta, p_idx, build):
         """Column p_idx of the LM Jacobian images from the shared rotation
-        chain c1 = Rz x, c2 = Ry c1, c3 = Rx c2 (built when ``build``).  c3
-        lives in the work buffer, so columns 0-2 come before 3-5."""
+        chain c1 = Rz x, c2 = Ry c1, c3 = Tr Rx c2 = T_d x (built when
+        ``build``).  c3 lives in the work buffer, so columns 0-2 (one -i k_p
+        pass over c3 each) come before 3-5."""
         g = self.geom
         B = theta.shape[0]
         if getattr(self, "_chain", None) is None:
