@@ -1147,8 +1147,8 @@ def calib_bench(args, torch, me, dev):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); shot sharding across GPUs is a design note, "
-              f"not part of this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); shot sharding across GPUs is designed but "
+              f"not implemented", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

@@ -3,7 +3,7 @@ extract_from_dat.py:6-60, simulation.py:476-493).
 
 * ``load_preprocessed``: the ksp / ref / noise arrays that the reference's
   TWIX extraction writes as .npy (reading the Siemens .dat itself needs
-  ``twixtools``, which is not part of this build).
+  ``twixtools``, a third-party reader this package does not include).
 * ``shot_masks_from_counters``: DISORDER shot masks from the scanner's
   line/partition counters (recon.py:158-165).
 * ``prepare_real_data``: the calibration chain of recon.py:130-150 on the

@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY.  Nothing in the product package imports, links or
 executes this module.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` use it, and only as the checker (or, for
+``cpu_baseline`` leg of ``bench.py`` use it, and only to check results (or, for
 ``cpu_baseline``, as the timed CPU restatement), never as the thing measured on
 the GPU.
 
