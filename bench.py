@@ -40,7 +40,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true",
                     help="skip the CPU-baseline leg: the oracle timing and the small-size FP64-oracle accuracy check")
     ap.add_argument("--no-objective", action="store_true", help="skip the per-iteration objective")
-    ap.add_argument("--no-tune", action="store_true", help="skip the line-pass autotuner at setup")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuners at setup (line passes and forward encoding; every level with --full)")
     ap.add_argument("--grad", action="store_true",
                     help="differentiation workload instead of the headline: data-consistency loss forward+backward "
                          "at c3 (192x192x128, 24 coils, 48 shots), recompute on/off, and a 64-direction central "
@@ -935,7 +936,7 @@ def full_bench(args, torch, me, dev):
     t0 = time.perf_counter()
     x, t_est = me.pyramid_reconstruction(data["ksp"], data["mps"], data["A"], nshots, n_spatial_levels=3,
                                          n_joint_iters=iters, final_joint_iters=args.final_iters,
-                                         compute_objective=False, progress=log)
+                                         compute_objective=False, progress=log, tune=not args.no_tune)
     total_s = time.perf_counter() - t0
     levels = me.pyramid.pyramid_reconstruction.levels
     log(f"pyramid done in {total_s:.1f}s: " + ", ".join(f"x{l['factor']} {l['seconds']:.1f}s" for l in levels))

@@ -75,7 +75,7 @@ def resize_mask(A, oshape):
 
 def pyramid_reconstruction(ksp, mps, A, nshots, n_spatial_levels=3, n_temporal_levels=3, n_joint_iters=100,
                            save_path=None, sampling="disorder", final_cg_iter=100, compute_objective=True,
-                           final_joint_iters=0, progress=None):
+                           final_joint_iters=0, progress=None, tune=False):
     """pyramid.py:24-118.  Returns (x centred numpy image, t_est numpy
     (nshots, 6)).  ``n_temporal_levels`` is unused, as in the reference.
     Convergence histories are written to ``save_path`` as
@@ -88,7 +88,19 @@ def pyramid_reconstruction(ksp, mps, A, nshots, n_spatial_levels=3, n_temporal_l
     motion are returned.  ``pyramid_reconstruction.levels`` records each
     level's factor, shape, objective history, transform history and wall
     time; ``progress`` (a callable taking a message) is told when each level
-    ends."""
+    ends.  ``tune=True`` runs the autotuners (line-pass lines per workgroup,
+    forward-encoding variant) on each level's geometry before its iterations;
+    results do not depend on the choice, only the time does."""
+    if tune:
+        from .batched import tune_encoders, tune_problem
+
+        def _tune(pb):
+            tune_problem(pb)
+            tune_encoders(pb)
+    else:
+        def _tune(pb):
+            pass
+
     dev = _lib.require_gpu()
     ksp = as_c64(ksp, dev)
     mps = as_c64(mps, dev)
@@ -114,6 +126,7 @@ def pyramid_reconstruction(ksp, mps, A, nshots, n_spatial_levels=3, n_temporal_l
         kgrid, rkgrid = compute_transform_grids_voxel(full, [1, 1, 1], ds)
         app = JointRecon(ksp_res, mps_res, A_res, kgrid, rkgrid, t0=t_est, max_joint_iter=int(n_joint_iters * f),
                          compute_objective=compute_objective)
+        _tune(app.problem)
         _, t = app.run()
         t_est = t.detach().cpu().numpy().copy()
         levels.append(dict(factor=f, shape=ds, objective=app.objective_history, transforms=app.transform_history,
@@ -129,6 +142,7 @@ def pyramid_reconstruction(ksp, mps, A, nshots, n_spatial_levels=3, n_temporal_l
     if final_joint_iters and final_joint_iters > 0:
         app = JointRecon(ksp, mps, A, kgrid, rkgrid, t0=t_est, max_joint_iter=int(final_joint_iters),
                          compute_objective=compute_objective)
+        _tune(app.problem)
         img, t = app.run()
         t_est = t.detach().cpu().numpy().copy()
         torch.cuda.synchronize(dev)
@@ -138,6 +152,7 @@ def pyramid_reconstruction(ksp, mps, A, nshots, n_spatial_levels=3, n_temporal_l
         return img.cpu().numpy(), t_est
     # full resolution: image estimation only (pyramid.py:96-115)
     prob = SenseProblem(ksp, mps, A, kgrid, rkgrid, centered=True, normalize=True)
+    _tune(prob)
     prob.theta.copy_(torch.as_tensor(t_est, device=dev))
     x = prob.zeros_image()
     b = prob.rhs()
