@@ -97,6 +97,26 @@ int me_rigid_deriv_apply(const void* x, long long x_bstride, const void* u, long
                          const double* r2, const double* theta, long long theta_stride, double* coef_ws, int p_idx,
                          int build, hipStream_t stream);
 
+/* The same columns reduced against w without storing them:
+ *   out[b * out_stride] = Re <w_b, column p_idx of entry b>   (FP64)
+ * The last line pass of the column sums Re(conj(w) col) per workgroup into
+ * `partials` (me_rigid_dot_partials_size doubles) and a fixed-order sum per
+ * entry follows.  y (one volume per entry) holds the rotation columns'
+ * intermediate passes (unused for p_idx < 3).  This is the motion gradient
+ * of a loss through RigidTransformCudaOptimzied.apply when w is the adjoint
+ * encoding of the residual; it corresponds to the reference's gradient
+ * entries g_p = Re <r, J_p> (transform_estimation.py:56), formed in image
+ * space. */
+int me_rigid_deriv_apply_dot(const void* x, long long x_bstride, const void* u, long long u_bstride, void* c1,
+                             void* c2, long long c_bstride, const void* w, long long w_bstride, void* y,
+                             long long y_bstride, void* work, long long work_bstride, int nbatch, const int* n,
+                             const long long* st, const double* r0, const double* r1, const double* r2,
+                             const double* theta, long long theta_stride, double* coef_ws, int p_idx, int build,
+                             double* partials, double* out, int out_stride, hipStream_t stream);
+
+/* Doubles of the `partials` workspace of me_rigid_deriv_apply_dot. */
+long long me_rigid_dot_partials_size(int nbatch, const int* n, const long long* st);
+
 /* Orthonormal 1-D FFT (inverse != 0: inverse FFT) along `axis` of a batch of
  * volumes; x and y may alias.  Replaces the cupyx.scipy.fft fft/ifft/fftn
  * calls used to prepare data (joint_recon.py:63-68, pyramid.py:65). */

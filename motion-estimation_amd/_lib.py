@@ -74,6 +74,8 @@ _SIGS = {
     "me_espirit_maps": [vp, i32, vp, vp, C.c_double, i32, vp, vp, vp],
     "me_rigid_deriv_apply": [vp, i64, vp, i64, vp, vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp,
                              i32, i32, vp],
+    "me_rigid_deriv_apply_dot": [vp, i64, vp, i64, vp, vp, i64, vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp,
+                                 dp, i64, dp, i32, i32, vp, vp, i32, vp],
     "me_mask_mul": [vp, vp, i64, vp],
     "me_lm_gram": [vp, vp, i64, vp, vp, i32, i32, vp, vp, vp],
     "me_lm_solve": [vp, vp, vp, i32, vp, vp, i32, vp],
@@ -94,6 +96,7 @@ _SIZES = {
     "me_encode_tuning_get": ([vp, i32, i32], C.c_int),
     "me_metric_partials_size": ([], C.c_int),
     "me_jacobi_workspace_size": ([i32], C.c_longlong),
+    "me_rigid_dot_partials_size": ([i32, vp, vp], C.c_longlong),
     "me_jacobi_set_lds_rows": ([i32], C.c_int),
     "me_espirit_maps_workspace_size": ([i32, vp, vp], C.c_longlong),
 }

@@ -12,8 +12,8 @@ torch scalar.  The backward is written out, not traced:
   the gathered residual, then the batched adjoint transform summed over shots);
 * motion: dL/dtheta_{s,p} = Re < E_s^H r_s, dT_s/dtheta_p x >, with the six
   Jacobian columns of the apply order built from shared rotation chains
-  (c1 = Rx x, c2 = Tr_z Ry c1, u = T x; me_rigid_deriv_apply) and reduced per
-  shot in FP64 (me_dot_re_batched).
+  (c1 = Rx x, c2 = Tr_z Ry c1, u = T x), each reduced per shot in FP64 by
+  its own last line pass instead of being stored (me_rigid_deriv_apply_dot).
 
 The sheared intermediates (u, c1, c2) are recomputed in the backward by
 default; ``recompute=False`` keeps u = T x from the forward instead (one
@@ -46,6 +46,23 @@ def _deriv_apply(pb, x, u, u_bs, c1, c2, out, work, theta, p_idx, build):
               _lib.ptr(out), pb.vol, _lib.ptr(work), pb.vol, int(B), g.n_arg(), g.st_arg(True), *g.r_args(),
               _lib.ptr(theta), int(theta.stride(0)), _lib.ptr(coef), int(p_idx), int(bool(build)),
               _lib.stream(pb.device))
+
+
+def _deriv_apply_dot(pb, x, u, u_bs, c1, c2, w, y, work, theta, p_idx, build, out, out_stride):
+    """out[b * out_stride] = Re <w_b, dT_b/dtheta_p x> (FP64): the column's
+    last line pass reduces against w instead of storing the column."""
+    g = pb.geom
+    B = theta.shape[0]
+    coef = torch.empty((B, 16), dtype=torch.float64, device=pb.device)
+    n = int(_lib.size("me_rigid_dot_partials_size", int(B), g.n_arg(), g.st_arg(True)))
+    part = getattr(pb, "_rdot_partials", None)
+    if part is None or part.numel() < n:
+        part = torch.empty(max(n, 1), dtype=torch.float64, device=pb.device)
+        pb._rdot_partials = part
+    _lib.call("me_rigid_deriv_apply_dot", _lib.ptr(x), 0, _lib.ptr(u), u_bs, _lib.ptr(c1), _lib.ptr(c2), pb.vol,
+              _lib.ptr(w), pb.vol, _lib.ptr(y), pb.vol, _lib.ptr(work), pb.vol, int(B), g.n_arg(), g.st_arg(True),
+              *g.r_args(), _lib.ptr(theta), int(theta.stride(0)), _lib.ptr(coef), int(p_idx), int(bool(build)),
+              _lib.ptr(part), _lib.ptr(out), int(out_stride), _lib.stream(pb.device))
 
 
 def _dots(pb, a, c, B, out, out_stride):
@@ -112,8 +129,8 @@ class DataConsistencyLoss(torch.autograd.Function):
                 else:
                     u = Ust[s0:s1]
                 for p in range(6):
-                    _deriv_apply(pb, xp, u, pb.vol, c1, c2, col, work, th[s0:s1], p, p == 3)
-                    _dots(pb, W, col, B, gt[s0:s1, p:], 6)
+                    _deriv_apply_dot(pb, xp, u, pb.vol, c1, c2, W, col, work, th[s0:s1], p, p == 3,
+                                     gt[s0:s1, p:], 6)
         g = grad_out.to(torch.float64)
         grad_x = grad_t = None
         if need_x:

@@ -98,8 +98,10 @@ __device__ __forceinline__ void phase_acc(const PhaseSpec& P, const double* cf, 
 // MODE 0: FFT, phase, inverse FFT; 1: forward FFT only (natural-order
 // spectrum out); 2: inverse FFT only (natural-order spectrum in).
 // REDUCE: sum the result over the batch and write batch entry 0 only.
-template <int N, bool CONTIG, int NIN, int NOUT, int MODE, bool REDUCE, int LL = kLinesDefault>
+// DOT: the dot epilogue (LineArgs::dot_w) instead of storing the output.
+template <int N, bool CONTIG, int NIN, int NOUT, int MODE, bool REDUCE, int LL = kLinesDefault, bool DOT = false>
 __global__ void __launch_bounds__((LineT<N, CONTIG, LL>::NT)) k_linepass(LineArgs A) {
+  static_assert(!DOT || (NOUT == 1 && MODE == 0 && !REDUCE), "dot epilogue: single-output MODE 0 passes");
   using LT = LineT<N, CONTIG, LL>;
   constexpr int R1 = LT::R1;
   constexpr int R2 = LT::R2;
@@ -142,6 +144,7 @@ __global__ void __launch_bounds__((LineT<N, CONTIG, LL>::NT)) k_linepass(LineArg
   const float out_scale = MODE == 0 ? 1.0f : rsqrtf((float)N);
   __syncthreads();
 
+  double dotv = 0.0;  // dot epilogue: this thread's share of Re(conj(w) out)
   const int b_begin = REDUCE ? 0 : (int)blockIdx.y;
   const int b_end = REDUCE ? A.nbatch : (int)blockIdx.y + 1;
   float2 red[REDUCE ? R1 : 1];
@@ -238,6 +241,16 @@ __global__ void __launch_bounds__((LineT<N, CONTIG, LL>::NT)) k_linepass(LineArg
         if (REDUCE) {
 #pragma unroll
           for (int n1 = 0; n1 < R1; ++n1) red[n1 < (REDUCE ? R1 : 1) ? n1 : 0] = c_add(red[n1 < (REDUCE ? R1 : 1) ? n1 : 0], a[n1]);
+        } else if (DOT) {
+          if (valid) {
+            const float2* wt = A.dot_w + (long long)b * A.dot_w_bs + line_off + (long long)t * A.st_a;
+            const long long step = (long long)R2 * A.st_a;
+#pragma unroll
+            for (int n1 = 0; n1 < R1; ++n1) {
+              const float2 w = wt[n1 * step];
+              dotv += (double)w.x * a[n1].x + (double)w.y * a[n1].y;
+            }
+          }
         } else if (valid) {
           float2* dt = dst + (long long)t * A.st_a;
           const long long step = (long long)R2 * A.st_a;
@@ -252,6 +265,18 @@ __global__ void __launch_bounds__((LineT<N, CONTIG, LL>::NT)) k_linepass(LineArg
         }
       }
       __syncthreads();
+    }
+  }
+  if (DOT) {  // one partial per workgroup
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dotv += __shfl_down(dotv, o, 64);
+    double* sh = reinterpret_cast<double*>(lds);  // free after the last barrier
+    if ((tid & 63) == 0) sh[tid >> 6] = dotv;
+    __syncthreads();
+    if (tid == 0) {
+      double sum = 0.0;
+      for (int w = 0; w < (NT + 63) / 64; ++w) sum += sh[w];
+      A.dot_partial[(long long)blockIdx.y * gridDim.x + blockIdx.x] = sum;
     }
   }
   if (REDUCE && t < R2 && valid) {
@@ -302,18 +327,22 @@ struct LineLaunch {
   dim3 grid;
   hipStream_t stream;
   int lines;
-  template <int N, int LL>
+  long long* grid_x;
+  template <int N, int LL, bool DOT = false>
   void go() {
     using LT = LineT<N, CONTIG, LL>;
     const long long nblk = CONTIG ? ((long long)args->n_u * args->n_v + LT::L - 1) / LT::L
                                   : (long long)((args->n_u + LT::L - 1) / LT::L) * args->n_v;
     dim3 g = grid;
     g.x = (unsigned)nblk;
-    hipLaunchKernelGGL((k_linepass<N, CONTIG, NIN, NOUT, MODE, REDUCE, LL>), g, dim3(LT::NT), 0, stream, *args);
+    if (grid_x) *grid_x = nblk;
+    hipLaunchKernelGGL((k_linepass<N, CONTIG, NIN, NOUT, MODE, REDUCE, LL, DOT>), g, dim3(LT::NT), 0, stream,
+                       *args);
   }
   template <int N>
   void operator()() {
     if constexpr (NIN == 1 && NOUT == 1 && MODE == 0 && !REDUCE) {
+      if (args->dot_w) return go<N, kLinesDefault, true>();
       if constexpr (lines_fit<N, CONTIG, 8>())
         if (lines == 8) return go<N, 8>();
       if constexpr (lines_fit<N, CONTIG, 32>())
@@ -324,20 +353,21 @@ struct LineLaunch {
 };
 
 template <bool CONTIG, int NIN, int NOUT, int MODE, bool REDUCE>
-bool launch_n(int n, const LineArgs& a, dim3 grid, hipStream_t s, int lines) {
-  LineLaunch<CONTIG, NIN, NOUT, MODE, REDUCE> f{&a, grid, s, lines};
+bool launch_n(int n, const LineArgs& a, dim3 grid, hipStream_t s, int lines, long long* grid_x) {
+  LineLaunch<CONTIG, NIN, NOUT, MODE, REDUCE> f{&a, grid, s, lines, grid_x};
   return dispatch_line_length(n, f);
 }
 
 template <bool CONTIG>
-bool launch_io(int n, int nin, int nout, const LineArgs& a, dim3 grid, hipStream_t s, int lines) {
-  if (a.mode == 1) return nin == 1 && nout == 1 && launch_n<CONTIG, 1, 1, 1, false>(n, a, grid, s, lines);
-  if (a.mode == 2) return nin == 1 && nout == 1 && launch_n<CONTIG, 1, 1, 2, false>(n, a, grid, s, lines);
-  if (a.reduce) return nin == 1 && nout == 1 && launch_n<CONTIG, 1, 1, 0, true>(n, a, grid, s, lines);
-  if (nin == 1 && nout == 1) return launch_n<CONTIG, 1, 1, 0, false>(n, a, grid, s, lines);
-  if (nin == 1 && nout == 2) return launch_n<CONTIG, 1, 2, 0, false>(n, a, grid, s, lines);
-  if (nin == 2 && nout == 2) return launch_n<CONTIG, 2, 2, 0, false>(n, a, grid, s, lines);
-  if (nin == 2 && nout == 1) return launch_n<CONTIG, 2, 1, 0, false>(n, a, grid, s, lines);
+bool launch_io(int n, int nin, int nout, const LineArgs& a, dim3 grid, hipStream_t s, int lines,
+               long long* gx) {
+  if (a.mode == 1) return nin == 1 && nout == 1 && launch_n<CONTIG, 1, 1, 1, false>(n, a, grid, s, lines, gx);
+  if (a.mode == 2) return nin == 1 && nout == 1 && launch_n<CONTIG, 1, 1, 2, false>(n, a, grid, s, lines, gx);
+  if (a.reduce) return nin == 1 && nout == 1 && launch_n<CONTIG, 1, 1, 0, true>(n, a, grid, s, lines, gx);
+  if (nin == 1 && nout == 1) return launch_n<CONTIG, 1, 1, 0, false>(n, a, grid, s, lines, gx);
+  if (nin == 1 && nout == 2) return launch_n<CONTIG, 1, 2, 0, false>(n, a, grid, s, lines, gx);
+  if (nin == 2 && nout == 2) return launch_n<CONTIG, 2, 2, 0, false>(n, a, grid, s, lines, gx);
+  if (nin == 2 && nout == 1) return launch_n<CONTIG, 2, 1, 0, false>(n, a, grid, s, lines, gx);
   return false;
 }
 }  // namespace
@@ -395,11 +425,13 @@ static bool lines_built(int n, bool contig, int lines) {
 }
 
 int launch_linepass(const VolGeom& g, int axis, int nin, int nout, LineArgs A, const HostPhase ph[2][2],
-                    hipStream_t stream) {
+                    hipStream_t stream, long long* grid_x) {
   ME_REQUIRE(axis >= 0 && axis < 3, "line pass: axis out of range");
   ME_REQUIRE(nin >= 1 && nin <= 2 && nout >= 1 && nout <= 2, "line pass: 1 or 2 inputs/outputs");
   ME_REQUIRE(!(A.reduce && (nout != 1 || nin != 1)), "line pass: batch reduction needs one input and output");
   ME_REQUIRE(A.mode == 0 || (nin == 1 && nout == 1 && !A.reduce), "line pass: FFT-only modes are single-buffer");
+  ME_REQUIRE(!A.dot_w || (A.mode == 0 && nin == 1 && nout == 1 && !A.reduce && A.dot_partial),
+             "line pass: the dot epilogue needs a one-input, one-output MODE 0 pass and a partials buffer");
   ME_REQUIRE(A.nbatch >= 1, "line pass: empty batch");
   int au, av;
   bool contig;
@@ -439,14 +471,23 @@ int launch_linepass(const VolGeom& g, int axis, int nin, int nout, LineArgs A, c
   dim3 grid(1u, A.reduce ? 1u : (unsigned)A.nbatch);
   ME_REQUIRE(A.reduce || A.nbatch <= 65535, "line pass: batch too large");
   const int lines = tuned_lines(LineKey{g.n[axis], contig ? 1 : 0, A.n_u, A.n_v, A.st_a, A.st_u, A.st_v});
-  const bool ok = contig ? launch_io<true>(g.n[axis], nin, nout, A, grid, stream, lines)
-                         : launch_io<false>(g.n[axis], nin, nout, A, grid, stream, lines);
+  const bool ok = contig ? launch_io<true>(g.n[axis], nin, nout, A, grid, stream, lines, grid_x)
+                         : launch_io<false>(g.n[axis], nin, nout, A, grid, stream, lines, grid_x);
   ME_REQUIRE(ok, "line pass: unsupported line length " + std::to_string(g.n[axis]) +
                      " (supported: 8,12,16,24,32,48,64,96,128,192,256,384,512)");
   ME_TRY_HIP(hipGetLastError());
   return 0;
 }
 
+
+long long linepass_max_tiles(const VolGeom& g, int axis) {
+  bool contig;
+  int au, av;
+  line_layout(g, axis, contig, au, av);
+  const long long lines = 8;  // the fewest lines per workgroup of any variant
+  return contig ? ((long long)g.n[au] * g.n[av] + lines - 1) / lines
+                : ((long long)(g.n[au] + lines - 1) / lines) * g.n[av];
+}
 
 static int tune_set(const VolGeom& g, int axis, int lines) {
   std::lock_guard<std::mutex> lk(g_tune_mu);

@@ -68,6 +68,13 @@ struct LineArgs {
   const double* r_v;
   const unsigned char* mask;  // optional output mask (same layout as out)
   long long mask_bs;
+  // Dot epilogue (single-output MODE 0 passes, not REDUCE): when dot_w is
+  // set the output is not stored; each workgroup writes the FP64 partial
+  // sum of Re(conj(dot_w) out) over its lines to
+  // dot_partial[batch * gridDim.x + blockIdx.x].
+  const float2* dot_w;
+  long long dot_w_bs;
+  double* dot_partial;
   PhaseSpec ph[2][2];  // [output][input]
 };
 
@@ -95,6 +102,11 @@ struct VolGeom {
 // carries buffers, batch strides, mode/flags and coefficients; geometry and
 // phases are filled in here.  Returns 0 or an error code (message set).
 int launch_linepass(const VolGeom& g, int axis, int nin, int nout, LineArgs args,
-                    const HostPhase ph[2][2], hipStream_t stream);
+                    const HostPhase ph[2][2], hipStream_t stream, long long* grid_x = nullptr);
+
+// Upper bound of the workgroups per batch entry of a pass along ``axis``
+// (the fewest lines per workgroup any variant uses): the size of a dot
+// epilogue's partials per batch entry.
+long long linepass_max_tiles(const VolGeom& g, int axis);
 
 }  // namespace me

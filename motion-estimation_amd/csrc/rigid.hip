@@ -11,6 +11,7 @@
 //          Rotation derivatives use the three-term product rule as three passes
 //          with 1->2, 2->2, 2->1 buffers; translation derivatives multiply the
 //          axis-p translation phase by -i k_p.
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -325,8 +326,19 @@ struct Bufs {
   long long w_bs;
 };
 
+// Dot epilogue of the last pass (see LineArgs::dot_w): its output is not
+// stored; the per-workgroup partials go to `partial` and their count per
+// batch entry to *grid_x.
+struct DotOut {
+  const float2* w = nullptr;
+  long long w_bs = 0;
+  double* partial = nullptr;
+  long long* grid_x = nullptr;
+};
+
 int run_passes(const std::vector<Pass>& passes, const VolGeom& g, Bufs b, int nbatch, const double* coef,
-               float2* sum_out, const unsigned char* mask, bool accumulate, hipStream_t stream) {
+               float2* sum_out, const unsigned char* mask, bool accumulate, hipStream_t stream,
+               const DotOut& dot = DotOut{}) {
   for (size_t i = 0; i < passes.size(); ++i) {
     const Pass& p = passes[i];
     LineArgs A{};
@@ -365,10 +377,31 @@ int run_passes(const std::vector<Pass>& passes, const VolGeom& g, Bufs b, int nb
       A.mask_bs = 0;
       A.accumulate = accumulate ? 1 : 0;
     }
-    int rc = launch_linepass(g, p.axis, nin, p.nout, A, ph, stream);
+    long long* gx = nullptr;
+    if (last && dot.w) {
+      A.dot_w = dot.w;
+      A.dot_w_bs = dot.w_bs;
+      A.dot_partial = dot.partial;
+      gx = dot.grid_x;
+    }
+    int rc = launch_linepass(g, p.axis, nin, p.nout, A, ph, stream, gx);
     if (rc) return rc;
   }
   return 0;
+}
+
+// out[b * out_stride] = sum of the nblk partials of batch entry b, in order.
+__global__ void k_dot_finalize(const double* __restrict__ partial, long long nblk, double* __restrict__ out,
+                               int out_stride) {
+  __shared__ double sh[4];
+  const int b = blockIdx.x;
+  double v = 0.0;
+  for (long long i = threadIdx.x; i < nblk; i += blockDim.x) v += partial[(long long)b * nblk + i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) out[(long long)b * out_stride] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
 int make_geom(VolGeom& g, const int* n, const long long* st, const double* r0, const double* r1,
@@ -492,6 +525,55 @@ int me_rigid_deriv_apply(const void* x, long long x_bstride, const void* u, long
   const long long fbs = p_idx <= 2 ? u_bstride : (p_idx == 3 ? x_bstride : c_bstride);
   Bufs b{(const float2*)from, fbs, 1, (float2*)y, y_bstride, (float2*)work, work_bstride};
   return run_passes(build_deriv_apply(p_idx), g, b, nbatch, coef_ws, nullptr, nullptr, false, stream);
+}
+
+long long me_rigid_dot_partials_size(int nbatch, const int* n, const long long* st) {
+  VolGeom g;
+  if (nbatch < 1 || make_geom(g, n, st, nullptr, nullptr, nullptr) != 0) return 0;
+  long long m = 0;
+  for (int a = 0; a < 3; ++a) m = std::max(m, linepass_max_tiles(g, a));
+  return m * nbatch;
+}
+
+int me_rigid_deriv_apply_dot(const void* x, long long x_bstride, const void* u, long long u_bstride, void* c1,
+                             void* c2, long long c_bstride, const void* w, long long w_bstride, void* y,
+                             long long y_bstride, void* work, long long work_bstride, int nbatch, const int* n,
+                             const long long* st, const double* r0, const double* r1, const double* r2,
+                             const double* theta, long long theta_stride, double* coef_ws, int p_idx, int build,
+                             double* partials, double* out, int out_stride, hipStream_t stream) {
+  VolGeom g;
+  if (int rc = make_geom(g, n, st, r0, r1, r2)) return rc;
+  ME_REQUIRE(w && partials && out && out_stride >= 1, "rigid_deriv_apply_dot: null w/partials/out");
+  ME_REQUIRE(p_idx >= 0 && p_idx <= 5, "rigid_deriv_apply_dot: p_idx must be in [0, 5]");
+  ME_REQUIRE(p_idx >= 3 || u, "rigid_deriv_apply_dot: translation columns need u = T x");
+  ME_REQUIRE(p_idx < 3 || (work && y), "rigid_deriv_apply_dot: rotation columns need y and work buffers");
+  ME_REQUIRE(!build || (x && c1 && c2), "rigid_deriv_apply_dot: building the chain needs x, c1 and c2");
+  ME_REQUIRE(p_idx != 3 || x, "rigid_deriv_apply_dot: column 3 needs x");
+  ME_REQUIRE(p_idx != 4 || c1, "rigid_deriv_apply_dot: column 4 needs c1");
+  ME_REQUIRE(p_idx != 5 || c2, "rigid_deriv_apply_dot: column 5 needs c2");
+  ME_REQUIRE(nbatch >= 1 && nbatch <= 65535, "rigid_deriv_apply_dot: bad batch count");
+  if (int rc = launch_coef(theta, theta_stride, coef_ws, nbatch, stream)) return rc;
+  if (build) {
+    Bufs b0{(const float2*)x, x_bstride, 1, (float2*)c1, c_bstride, nullptr, 0};
+    if (int rc = run_passes(build_apply_chain(0), g, b0, nbatch, coef_ws, nullptr, nullptr, false, stream)) return rc;
+    Bufs b1{(const float2*)c1, c_bstride, 1, (float2*)c2, c_bstride, nullptr, 0};
+    if (int rc = run_passes(build_apply_chain(1), g, b1, nbatch, coef_ws, nullptr, nullptr, false, stream)) return rc;
+  }
+  const void* from = p_idx <= 2 ? u : (p_idx == 3 ? x : (p_idx == 4 ? (const void*)c1 : (const void*)c2));
+  const long long fbs = p_idx <= 2 ? u_bstride : (p_idx == 3 ? x_bstride : c_bstride);
+  Bufs b{(const float2*)from, fbs, 1, (float2*)y, y_bstride, (float2*)work, work_bstride};
+  long long nblk = 0;
+  DotOut dot;
+  dot.w = (const float2*)w;
+  dot.w_bs = w_bstride;
+  dot.partial = partials;
+  dot.grid_x = &nblk;
+  if (int rc = run_passes(build_deriv_apply(p_idx), g, b, nbatch, coef_ws, nullptr, nullptr, false, stream, dot))
+    return rc;
+  ME_REQUIRE(nblk > 0, "rigid_deriv_apply_dot: no workgroups");
+  hipLaunchKernelGGL(k_dot_finalize, dim3(nbatch), dim3(256), 0, stream, partials, nblk, out, out_stride);
+  ME_TRY_HIP(hipGetLastError());
+  return 0;
 }
 
 int me_fft_axis(const void* x, long long x_bstride, void* y, long long y_bstride, int nbatch, const int* n,

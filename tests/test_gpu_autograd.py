@@ -11,6 +11,8 @@
   quotient of FP32 data is noisy for directions nearly orthogonal to the
   gradient, where the plain relative error is not meaningful).
 * recompute=True and recompute=False give identical gradients.
+* The columns' dots formed inside their last line pass equal the stored
+  columns' FP64 dots (to FP32 rounding of the columns).
 """
 import numpy as np
 import pytest
@@ -61,6 +63,29 @@ def test_apply_order_columns_match_oracle(me, dc):
         for s in (0, 5):
             ref = O.rigid_deriv_apply(dc["x"], dc["t"][s], dc["okg"], dc["orkg"], p)
             assert _rel(pb.image_from_plane(col[s]), ref) < 1e-4, (p, s)
+
+
+def test_column_dots_without_storing_the_columns(me, dc):
+    """me_rigid_deriv_apply_dot (the column's last pass reduces against w)
+    equals storing the column and taking the FP64 batched dot, for every p.
+    The two last passes are separately compiled kernels whose FP32 operation
+    sequences (fused multiply-adds) may differ, so the columns agree to FP32
+    rounding and their dots to about 1e-7 of the largest."""
+    pb = dc["pb"]
+    th = torch.as_tensor(dc["t"], device=pb.device)
+    xp = pb.image_to_plane(dc["x"])
+    B, vol = pb.nshots, pb.vol
+    bufs = torch.empty((6, B, vol), dtype=torch.complex64, device=pb.device)
+    u, c1, c2, col, work, w = bufs
+    w.copy_(torch.randn((B, vol), dtype=torch.complex64, device=pb.device))
+    pb._rigid(xp, 0, u, th, 0)
+    fused = torch.zeros((B, 6), dtype=torch.float64, device=pb.device)
+    plain = torch.zeros((B, 6), dtype=torch.float64, device=pb.device)
+    for p in range(6):
+        me.autograd._deriv_apply(pb, xp, u, vol, c1, c2, col, work, th, p, p == 3)
+        me.autograd._dots(pb, w, col, B, plain[:, p:], 6)
+        me.autograd._deriv_apply_dot(pb, xp, u, vol, c1, c2, w, col, work, th, p, p == 3, fused[:, p:], 6)
+    assert float((fused - plain).abs().max() / plain.abs().max()) < 1e-6
 
 
 def test_loss_and_gradients_match_oracle(me, dc):
