@@ -97,6 +97,15 @@ int me_rigid_deriv_apply(const void* x, long long x_bstride, const void* u, long
                          const double* r2, const double* theta, long long theta_stride, double* coef_ws, int p_idx,
                          int build, hipStream_t stream);
 
+/* Forward transform that keeps the apply-order chain for
+ * me_rigid_deriv_apply / _dot with build = 0: c1 = Rx x, c2 = Tr_z Ry c1 and
+ * u = T x = Tr_y (Rz Tr_x) c2, the passes of me_rigid_transform mode 0
+ * (transform.py:263-318) in its order, so u equals its output bit for bit. */
+int me_rigid_apply_chain(const void* x, long long x_bstride, void* c1, void* c2, long long c_bstride, void* u,
+                         long long u_bstride, int nbatch, const int* n, const long long* st, const double* r0,
+                         const double* r1, const double* r2, const double* theta, long long theta_stride,
+                         double* coef_ws, hipStream_t stream);
+
 /* The same columns reduced against w without storing them:
  *   out[b * out_stride] = Re <w_b, column p_idx of entry b>   (FP64)
  * The last line pass of the column sums Re(conj(w) col) per workgroup into

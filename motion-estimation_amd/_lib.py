@@ -76,6 +76,7 @@ _SIGS = {
                              i32, i32, vp],
     "me_rigid_deriv_apply_dot": [vp, i64, vp, i64, vp, vp, i64, vp, i64, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp,
                                  dp, i64, dp, i32, i32, vp, vp, i32, vp],
+    "me_rigid_apply_chain": [vp, i64, vp, vp, i64, vp, i64, i32, vp, vp, dp, dp, dp, dp, i64, dp, vp],
     "me_mask_mul": [vp, vp, i64, vp],
     "me_lm_gram": [vp, vp, i64, vp, vp, i32, i32, vp, vp, vp],
     "me_lm_solve": [vp, vp, vp, i32, vp, vp, i32, vp],

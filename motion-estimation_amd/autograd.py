@@ -16,8 +16,9 @@ torch scalar.  The backward is written out, not traced:
   its own last line pass instead of being stored (me_rigid_deriv_apply_dot).
 
 The sheared intermediates (u, c1, c2) are recomputed in the backward by
-default; ``recompute=False`` keeps u = T x from the forward instead (one
-volume per shot).  Inputs follow ``estimate_transform``'s frame (uncentred
+default (u through the chain the columns start from, so the chain is built
+once); ``recompute=False`` keeps u, c1 and c2 from the forward instead (three
+volumes per shot), whose transform then runs through the same chain.  Inputs follow ``estimate_transform``'s frame (uncentred
 ksp/mps/masks) through a :class:`SenseProblem`.
 """
 from __future__ import annotations
@@ -46,6 +47,17 @@ def _deriv_apply(pb, x, u, u_bs, c1, c2, out, work, theta, p_idx, build):
               _lib.ptr(out), pb.vol, _lib.ptr(work), pb.vol, int(B), g.n_arg(), g.st_arg(True), *g.r_args(),
               _lib.ptr(theta), int(theta.stride(0)), _lib.ptr(coef), int(p_idx), int(bool(build)),
               _lib.stream(pb.device))
+
+
+def _apply_chain(pb, x, c1, c2, u, theta):
+    """u = T x (bit for bit me_rigid_transform's apply) keeping the chain
+    c1 = Rx x, c2 = Tr_z Ry c1 that the Jacobian columns start from."""
+    g = pb.geom
+    B = theta.shape[0]
+    coef = torch.empty((B, 16), dtype=torch.float64, device=pb.device)
+    _lib.call("me_rigid_apply_chain", _lib.ptr(x), 0, _lib.ptr(c1), _lib.ptr(c2), pb.vol, _lib.ptr(u), pb.vol,
+              int(B), g.n_arg(), g.st_arg(True), *g.r_args(), _lib.ptr(theta), int(theta.stride(0)),
+              _lib.ptr(coef), _lib.stream(pb.device))
 
 
 def _deriv_apply_dot(pb, x, u, u_bs, c1, c2, w, y, work, theta, p_idx, build, out, out_stride):
@@ -87,11 +99,16 @@ class DataConsistencyLoss(torch.autograd.Function):
         th = theta.detach().to(device=dev, dtype=torch.float64).reshape(pb.nshots, 6).contiguous()
         R = torch.empty_like(pb.ymeas)
         norms = torch.empty(pb.nshots, dtype=torch.float64, device=dev)
-        U = None if recompute else torch.empty((pb.nshots, pb.vol), dtype=torch.complex64, device=dev)
+        # recompute=False keeps u = T x and its chain (c1, c2) for the backward
+        K = None if recompute else torch.empty((3, pb.nshots, pb.vol), dtype=torch.complex64, device=dev)
         ub = _bufs(pb, 1)[0]
         for s0, s1 in pb.batches():
-            dst = ub if recompute else U[s0:s1]
-            pb._rigid(xp, 0, dst, th[s0:s1], 0)
+            if recompute:
+                dst = ub
+                pb._rigid(xp, 0, dst, th[s0:s1], 0)
+            else:
+                dst = K[0, s0:s1]
+                _apply_chain(pb, xp, K[1, s0:s1], K[2, s0:s1], dst, th[s0:s1])
             pb._encode(dst, pb.vol, s0, s1, y=R, meas=True, norms=norms[s0:s1])
         ctx.problem = pb
         ctx.recompute = recompute
@@ -101,7 +118,7 @@ class DataConsistencyLoss(torch.autograd.Function):
         if recompute:
             ctx.save_for_backward(xp, th, R)
         else:
-            ctx.save_for_backward(xp, th, R, U)
+            ctx.save_for_backward(xp, th, R, K)
         return 0.5 * norms.sum()
 
     @staticmethod
@@ -109,7 +126,7 @@ class DataConsistencyLoss(torch.autograd.Function):
         pb = ctx.problem
         saved = ctx.saved_tensors
         xp, th, R = saved[:3]
-        Ust = None if ctx.recompute else saved[3]
+        Kst = None if ctx.recompute else saved[3]
         need_x, need_t = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
         gx = pb.zeros_image()
         gt = torch.zeros((pb.nshots, 6), dtype=torch.float64, device=pb.device)
@@ -123,13 +140,13 @@ class DataConsistencyLoss(torch.autograd.Function):
                 pb._rigid(W, pb.vol * pb.ncg, gx, th[s0:s1], 1, flags=1 | (4 if s0 > 0 else 0), work=work,
                           x_parts=pb.parts)
             if need_t:
-                if Ust is None:
-                    pb._rigid(xp, 0, U, th[s0:s1], 0)
-                    u = U
+                if Kst is None:  # u = T x through the chain the columns start from
+                    u, k1, k2 = U, c1, c2
+                    _apply_chain(pb, xp, k1, k2, u, th[s0:s1])
                 else:
-                    u = Ust[s0:s1]
+                    u, k1, k2 = Kst[0, s0:s1], Kst[1, s0:s1], Kst[2, s0:s1]
                 for p in range(6):
-                    _deriv_apply_dot(pb, xp, u, pb.vol, c1, c2, W, col, work, th[s0:s1], p, p == 3,
+                    _deriv_apply_dot(pb, xp, u, pb.vol, k1, k2, W, col, work, th[s0:s1], p, False,
                                      gt[s0:s1, p:], 6)
         g = grad_out.to(torch.float64)
         grad_x = grad_t = None

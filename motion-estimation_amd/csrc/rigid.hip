@@ -285,12 +285,19 @@ std::vector<Pass> build_deriv_from_chain(int p) {
 //   p = 4:     Tr_y (Rz Tr_x) d(Tr_z Ry) c1        (7 passes)
 //   p = 5:     Tr_y d(Tr_x Rz) c2                  (4 passes)
 // so the six columns take 3 + 3 + 3 + 10 + 7 + 4 = 30 passes given u.
-std::vector<Pass> build_apply_chain(int level) {  // level 0: c1 = Rx x; 1: c2 = Tr_z Ry c1
+// level 0: c1 = Rx x; 1: c2 = Tr_z Ry c1; 2: u = Tr_y (Rz Tr_x) c2 = T x.
+// The three levels are build_apply(false)'s passes in its order, so u equals
+// me_rigid_transform's apply bit for bit.
+std::vector<Pass> build_apply_chain(int level) {
   std::vector<Pass> v;
-  if (level == 0)
+  if (level == 0) {
     add_rotation(v, 0, false, -1, false, false, false);
-  else
+  } else if (level == 1) {
     add_rotation(v, 1, false, 2, false, false, false);
+  } else {
+    add_rotation(v, 2, false, 0, false, false, false);
+    v.push_back(single(1, trans(1, false)));
+  }
   return v;
 }
 
@@ -525,6 +532,26 @@ int me_rigid_deriv_apply(const void* x, long long x_bstride, const void* u, long
   const long long fbs = p_idx <= 2 ? u_bstride : (p_idx == 3 ? x_bstride : c_bstride);
   Bufs b{(const float2*)from, fbs, 1, (float2*)y, y_bstride, (float2*)work, work_bstride};
   return run_passes(build_deriv_apply(p_idx), g, b, nbatch, coef_ws, nullptr, nullptr, false, stream);
+}
+
+int me_rigid_apply_chain(const void* x, long long x_bstride, void* c1, void* c2, long long c_bstride, void* u,
+                         long long u_bstride, int nbatch, const int* n, const long long* st, const double* r0,
+                         const double* r1, const double* r2, const double* theta, long long theta_stride,
+                         double* coef_ws, hipStream_t stream) {
+  VolGeom g;
+  if (int rc = make_geom(g, n, st, r0, r1, r2)) return rc;
+  ME_REQUIRE(x && c1 && c2 && u, "rigid_apply_chain: null buffer");
+  ME_REQUIRE(nbatch >= 1 && nbatch <= 65535, "rigid_apply_chain: bad batch count");
+  if (int rc = launch_coef(theta, theta_stride, coef_ws, nbatch, stream)) return rc;
+  const void* src[3] = {x, c1, c2};
+  const long long sbs[3] = {x_bstride, c_bstride, c_bstride};
+  void* dst[3] = {c1, c2, u};
+  const long long dbs[3] = {c_bstride, c_bstride, u_bstride};
+  for (int l = 0; l < 3; ++l) {
+    Bufs b{(const float2*)src[l], sbs[l], 1, (float2*)dst[l], dbs[l], nullptr, 0};
+    if (int rc = run_passes(build_apply_chain(l), g, b, nbatch, coef_ws, nullptr, nullptr, false, stream)) return rc;
+  }
+  return 0;
 }
 
 long long me_rigid_dot_partials_size(int nbatch, const int* n, const long long* st) {

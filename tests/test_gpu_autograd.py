@@ -12,7 +12,8 @@
   gradient, where the plain relative error is not meaningful).
 * recompute=True and recompute=False give identical gradients.
 * The columns' dots formed inside their last line pass equal the stored
-  columns' FP64 dots (to FP32 rounding of the columns).
+  columns' FP64 dots (to FP32 rounding of the columns); the forward
+  transform built through the chain equals the plain apply bit for bit.
 """
 import numpy as np
 import pytest
@@ -63,6 +64,24 @@ def test_apply_order_columns_match_oracle(me, dc):
         for s in (0, 5):
             ref = O.rigid_deriv_apply(dc["x"], dc["t"][s], dc["okg"], dc["orkg"], p)
             assert _rel(pb.image_from_plane(col[s]), ref) < 1e-4, (p, s)
+
+
+def test_apply_chain_equals_apply(me, dc):
+    """me_rigid_apply_chain's u is me_rigid_transform's apply bit for bit
+    (the same passes in the same order), and its c1, c2 are the chain the
+    columns start from."""
+    pb = dc["pb"]
+    th = torch.as_tensor(dc["t"], device=pb.device)
+    xp = pb.image_to_plane(dc["x"])
+    B, vol = pb.nshots, pb.vol
+    bufs = torch.empty((7, B, vol), dtype=torch.complex64, device=pb.device)
+    ref, u, c1, c2, d1, d2, work = bufs
+    pb._rigid(xp, 0, ref, th, 0)
+    me.autograd._apply_chain(pb, xp, c1, c2, u, th)
+    assert torch.equal(u, ref)
+    col = torch.empty_like(u)
+    me.autograd._deriv_apply(pb, xp, u, vol, d1, d2, col, work, th, 3, True)  # builds its own chain
+    assert torch.equal(d1, c1) and torch.equal(d2, c2)
 
 
 def test_column_dots_without_storing_the_columns(me, dc):
