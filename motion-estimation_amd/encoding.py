@@ -330,18 +330,25 @@ class SenseProblem:
         self._rigid(parts, self.vol * self.ncg, out, theta[s0:s1], 1, flags=flags, work=self._wbuf(), mask=self.M,
                     x_parts=self.parts)
 
-    def normal(self, p, out, theta=None, y=None, norms=None):
+    def normal(self, p, out, theta=None, y=None, norms=None, residual=False):
         """out = M * sum_s T_s^H E_s^H E_s T_s p  (image_estimation.py:106-129).
 
         The gathered encodings E_s T_s p are left in ``y`` (default: the
         problem's sample buffer); with ``norms`` (nshots FP64) the per-shot
-        ||E_s T_s p - y_s||^2 come out of the same encoding."""
+        ||E_s T_s p - y_s||^2 come out of the same encoding.  ``residual``:
+        the encodings are taken against the measured data, so ``y`` holds
+        E_s T_s p - y_s and out = M * sum_s T_s^H E_s^H (E_s T_s p - y_s), which
+        is A p - b with b the right-hand side of image_estimation.py:92-103
+        (the encoding and the sums are linear), without a separate b."""
         theta = self.theta if theta is None else theta
         U = self._ubuf()
         Y = self._ybuf() if y is None else y
         for s0, s1 in self.batches():
             self._rigid(p, 0, U, theta[s0:s1], 0)
-            if norms is None:
+            if residual:
+                self._encode(U, self.vol, s0, s1, y=Y, meas=True,
+                             norms=None if norms is None else norms[s0:s1])
+            elif norms is None:
                 self._encode(U, self.vol, s0, s1, y=Y)
             else:
                 self._encode(U, self.vol, s0, s1, y=Y, meas=True, norms=norms[s0:s1], store_raw=True)
@@ -372,18 +379,30 @@ class SenseProblem:
     def cg(self, x, b, max_iter, tol=1e-6, check_every=1, theta=None, track=None, norms0=None):
         """PCG of image_estimation.py:130-165 (x updated in place).
 
-        ``track`` (a sample-sized buffer) receives E T x of the final x: the
-        initial normal operator writes E T x0 there and every step adds
-        alpha E T p, the encodings the normal operator of p already made.
-        ``norms0`` (nshots FP64) receives ||E_s T_s x0 - y_s||^2 of the
-        initial x from the same first encoding."""
+        ``b=None`` takes the right-hand side of image_estimation.py:92-103
+        implicitly: the initial residual r0 = b - A x0 is formed as
+        -M sum_s T_s^H E_s^H (E_s T_s x0 - y_s) in one normal operator, which
+        saves b's own adjoint transforms.
+
+        ``track`` (a sample-sized buffer) follows E T x through the steps: the
+        initial normal operator writes E T x0 there (E T x0 - y with
+        ``b=None``) and every step adds alpha E T p, the encodings the normal
+        operator of p already made, so it ends as E T x (or E T x - y) of the
+        final x.  ``norms0`` (nshots FP64) receives ||E_s T_s x0 - y_s||^2 of
+        the initial x from the same first encoding."""
         n = self.vol
         if getattr(self, "_cgv", None) is None:
             self._cgv = torch.empty((3, n), dtype=torch.complex64, device=self.device)
         Ap, r, p = self._cgv[0], self._cgv[1], self._cgv[2]
         st = self._cg_state
         s = _lib.stream(self.device)
-        self.normal(x, Ap, theta, y=track, norms=norms0)
+        residual = b is None
+        self.normal(x, Ap, theta, y=track, norms=norms0, residual=residual)
+        if residual:  # r = 0 - (A x0 - b)
+            if getattr(self, "_zero_img", None) is None:
+                self._zero_img = self.zeros_image()
+            b = self._zero_img
+        self._track_is_residual = residual and track is not None
         _lib.call("me_cg_init", _lib.ptr(b), _lib.ptr(Ap), _lib.ptr(r), _lib.ptr(p), _lib.ptr(self.P), n,
                   _lib.ptr(st), _lib.ptr(self._vec_partials), s)
         Y = self._ybuf()
@@ -422,9 +441,9 @@ class SenseProblem:
         Updates self.theta/damp/flags in place, returns T(mean) x (written to
         ``out`` when given, which may be x itself).
 
-        ``tracked``: the track buffer already holds E T x for this x and theta
-        (from cg(track=...)), so the residual is formed from it instead of a
-        new transform and encoding.  ``use_ga``: the reference's experimental
+        ``tracked``: the track buffer already holds E T x (or, after
+        cg(b=None, track=...), E T x - y) for this x and theta, so the residual
+        is formed from it instead of a new transform and encoding.  ``use_ga``: the reference's experimental
         geodesic acceleration (transform_estimation.py:73-121): six more
         columns at theta + 1e-4 v per shot and a second 6x6 solve."""
         dev = self.device
@@ -443,8 +462,9 @@ class SenseProblem:
         s = _lib.stream(dev)
         n0, n1, n2 = self.shape
         th = self.theta
-        if tracked:
+        if tracked and not getattr(self, "_track_is_residual", False):
             _lib.call("me_csub", _lib.ptr(R), _lib.ptr(R), _lib.ptr(self.ymeas), nY, s)
+        self._track_is_residual = False  # the buffer is reused below
         for s0, s1 in self.batches():
             B = s1 - s0
             if not tracked:

@@ -21,6 +21,5 @@ def estimate_image_cg(ksp, mps, shot_mask, transforms, kgrid, rkgrid, P, M, x0=N
     prob = SenseProblem(ksp, mps, shot_mask, kgrid, rkgrid, P=P, M=M, centered=False, normalize=False)
     prob.theta.copy_(as_f64(transforms, prob.device).reshape(prob.nshots, 6))
     x = prob.zeros_image() if x0 is None else prob.image_to_plane(x0)
-    b = prob.rhs(cache=False)
-    prob.cg(x, b, int(max_iter), float(tol))
+    prob.cg(x, None, int(max_iter), float(tol))  # b - A x0 in one normal operator
     return prob.image_from_plane(x)

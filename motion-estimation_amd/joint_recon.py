@@ -57,17 +57,18 @@ class JointRecon:
     def step(self, prev_objective=None):
         """One outer iteration (joint_recon.py:92-100).
 
-        The CG tracks E T x, so the first LM residual needs no new transform
-        and encoding.  ``prev_objective`` (nshots FP64 device tensor, or None)
-        receives the per-shot objective terms of the state this step starts
-        from, i.e. the previous iteration's objective (joint_recon.py:102),
-        computed from the CG's first encoding of that same state."""
+        The CG forms its initial residual b - A x0 from one normal operator of
+        E T x0 - y (no separate right-hand side) and tracks E T x - y, so the
+        first LM residual needs no new transform and encoding.
+        ``prev_objective`` (nshots FP64 device tensor, or None) receives the
+        per-shot objective terms of the state this step starts from, i.e. the
+        previous iteration's objective (joint_recon.py:102), computed from the
+        CG's first encoding of that same state."""
         pb = self.problem
-        if getattr(self, "_b", None) is None:
-            self._b = pb.zeros_image()
-        b = pb.rhs(out=self._b)
         track = pb.track_buffer() if self.max_nm_iter > 0 else None
-        pb.cg(self.x, b, self.max_cg_iter, 1e-6, check_every=self.cg_check_every, track=track,
+        # b = None: r0 = b - A x0 from one normal operator of the residual
+        # (estimate_image_cg's b, image_estimation.py:92-103, implicitly)
+        pb.cg(self.x, None, self.max_cg_iter, 1e-6, check_every=self.cg_check_every, track=track,
               norms0=prev_objective)
         for i in range(self.max_nm_iter):
             pb.lm_step(self.x, tracked=(i == 0), out=self.x)

@@ -155,8 +155,7 @@ def pyramid_reconstruction(ksp, mps, A, nshots, n_spatial_levels=3, n_temporal_l
     _tune(prob)
     prob.theta.copy_(torch.as_tensor(t_est, device=dev))
     x = prob.zeros_image()
-    b = prob.rhs()
-    prob.cg(x, b, int(final_cg_iter), 1e-6, check_every=1)
+    prob.cg(x, None, int(final_cg_iter), 1e-6, check_every=1)  # b - A x0 in one normal operator
     img = prob.image_from_plane(x, shift="f", scale=prob.max_norm)
     torch.cuda.synchronize(dev)
     levels.append(dict(factor=1, shape=full, objective=None, transforms=None, seconds=time.perf_counter() - t_level))

@@ -273,6 +273,36 @@ def test_cg_tracking_and_initial_norms(me, prob_data):
     assert _rel(track, fresh.cpu().numpy()) < TOL
 
 
+def test_cg_residual_form_matches_explicit_rhs(me, prob_data):
+    """cg(b=None) forms r0 = b - A x0 from one normal operator of
+    E T x0 - y: the iterates equal those with the explicit right-hand side,
+    the tracked buffer ends as the residual E T x - y of the final x, and
+    estimate_image_cg (which uses it) matches the oracle."""
+    d = prob_data
+    rng = np.random.default_rng(13)
+    x0 = 0.1 * (rng.standard_normal(SHAPE) + 1j * rng.standard_normal(SHAPE))
+    pb = _problem(me, d)
+    pb.theta.copy_(torch.as_tensor(d["t_true"]))
+    xa = pb.image_to_plane(x0)
+    pb.cg(xa, pb.rhs(), 4, 1e-12, check_every=0)
+    pb2 = _problem(me, d)
+    pb2.theta.copy_(torch.as_tensor(d["t_true"]))
+    xb = pb2.image_to_plane(x0)
+    track = pb2.track_buffer()
+    norms0 = torch.empty(pb2.nshots, dtype=torch.float64, device=pb2.device)
+    pb2.cg(xb, None, 4, 1e-12, check_every=0, track=track, norms0=norms0)
+    assert _rel(xb, xa.cpu().numpy()) < 1e-5
+    np.testing.assert_allclose(norms0.cpu().numpy(), pb2.objective_per_shot(pb2.image_to_plane(x0)).cpu().numpy(),
+                               rtol=1e-6)
+    fresh = pb2.forward(xb, meas=True).clone()
+    assert _rel(track, fresh.cpu().numpy()) < TOL
+    got = me.estimate_image_cg(d["ksp_u"], d["mps_u"], d["A_u"], d["t_true"], d["kg"], d["rkg"], d["P"], d["M"],
+                               x0=x0, max_iter=5)
+    ref = O.estimate_image_cg(d["ksp_u"], d["mps_u"], d["A_u"], d["t_true"], d["okg"], d["orkg"], d["P"], d["M"],
+                              x0=x0, max_iter=5)
+    assert _rel(got, ref) < TOL
+
+
 def test_empty_shot_matches_oracle(me, prob_data):
     """A shot with no samples (a user mask can hold one): it contributes
     nothing to the normal operator and right-hand side, and its LM step is
