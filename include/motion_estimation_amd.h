@@ -221,10 +221,10 @@ int me_encode_forward_static(const void* u, long long u_bstride, const void* map
 
 /* Forward-encoding variants and their tuning (no counterpart in the
  * reference).  k_encode_fwd runs G coils per workgroup with SPT samples per
- * thread: variant 0 = 16 x 2 (default), 1 = 8 x 2, 2 = 8 x 4; fewer coils per
- * group waste fewer threads when the coil count is not a multiple of 16 but
- * split large shots into more sample chunks, each of which redoes the row
- * FFTs.  The choice is kept in a process-wide table keyed by (N0, N1, N2,
+ * thread: variant 0 = 16 x 2 (default), 1 = 8 x 2, 2 = 8 x 4, 3 = 4 x 4; fewer
+ * coils per group waste fewer threads when the coil count is not a multiple
+ * of 16 but split large shots into more sample chunks, each of which redoes
+ * the row FFTs.  The choice is kept in a process-wide table keyed by (N0, N1, N2,
  * ncoils, max_cnt).  The samples are bitwise the same for every variant; the
  * FP64 norms agree to rounding (their per-workgroup partial sums group the
  * samples differently).  me_encode_partials_size covers every variant. */
@@ -234,7 +234,7 @@ int me_encode_tuning_get(const int* n, int ncoils, int max_cnt);
 void me_encode_tuning_clear(void);
 /* Times `reps` launches of me_encode_forward (no residual, no norms; y is
  * overwritten) for each variant with HIP events on `stream`, keeps the
- * fastest in the table; chosen (nullable) receives it, times_ms[3]
+ * fastest in the table; chosen (nullable) receives it, times_ms[4]
  * (nullable) the mean time per launch of each.  Synchronises the stream;
  * not for use under graph capture. */
 int me_tune_encode_forward(const void* u, long long u_bstride, const void* maps, int ncoils, int nbatch, const int* n,

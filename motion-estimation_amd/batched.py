@@ -57,7 +57,7 @@ def tune_encoders(pb, reps=3):
         if maxc in out:
             continue
         chosen = C.c_int()
-        times = (C.c_float * 3)()
+        times = (C.c_float * 4)()
         _lib.call("me_tune_encode_forward", _lib.ptr(U), pb.vol, _lib.ptr(pb.maps), pb.ncoils, s1 - s0,
                   _lib.int3(pb.shape), _lib.ptr(pl.sk0), _lib.ptr(pl.sk1), _lib.ptr(pl.off[s0:s1]),
                   _lib.ptr(pl.cnt[s0:s1]), maxc, _lib.ptr(Y), int(reps), C.byref(chosen), times,

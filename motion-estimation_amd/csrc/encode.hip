@@ -52,8 +52,8 @@ struct RowT {
 struct FwdVariant {
   int G, SPT;
 };
-constexpr FwdVariant kFwdVariants[] = {{16, 2}, {8, 2}, {8, 4}};
-constexpr int kNumFwdVariants = 3;
+constexpr FwdVariant kFwdVariants[] = {{16, 2}, {8, 2}, {8, 4}, {4, 4}};
+constexpr int kNumFwdVariants = 4;
 
 template <int NR, int G>
 struct FwdT {
@@ -826,6 +826,7 @@ bool launch_fwd(int variant, int n1, const EncArgs* a, dim3 grid, hipStream_t s)
     case 0: { FwdLaunch<2, 16> f{a, grid, s}; return dispatch_line_length(n1, f); }
     case 1: { FwdLaunch<2, 8> f{a, grid, s}; return dispatch_line_length(n1, f); }
     case 2: { FwdLaunch<4, 8> f{a, grid, s}; return dispatch_line_length(n1, f); }
+    case 3: { FwdLaunch<4, 4> f{a, grid, s}; return dispatch_line_length(n1, f); }
     default: return false;
   }
 }
@@ -986,7 +987,7 @@ long long me_encode_partials_size(int n2, int ncoils, int max_cnt, int n1) {
 
 int me_encode_tuning_set(const int* n, int ncoils, int max_cnt, int variant) {
   ME_REQUIRE(n && ncoils >= 1, "encode_tuning_set: bad argument");
-  ME_REQUIRE(variant >= 0 && variant < kNumFwdVariants, "encode_tuning_set: variant must be 0, 1 or 2");
+  ME_REQUIRE(variant >= 0 && variant < kNumFwdVariants, "encode_tuning_set: variant must be 0, 1, 2 or 3");
   std::lock_guard<std::mutex> lk(g_enc_mu);
   const auto key = std::make_tuple(n[0], n[1], n[2], ncoils, max_cnt);
   if (variant == 0)

@@ -76,7 +76,7 @@ def test_encoder_tuning_table_without_device(me):
     assert lib.me_encode_tuning_set(n, 9, 256, 2) == 0
     assert lib.me_encode_tuning_get(n, 9, 256) == 2
     assert lib.me_encode_tuning_get(n, 10, 256) == 0
-    assert lib.me_encode_tuning_set(n, 9, 256, 3) != 0 and b"variant" in lib.me_last_error()
+    assert lib.me_encode_tuning_set(n, 9, 256, 4) != 0 and b"variant" in lib.me_last_error()
     lib.me_encode_tuning_clear()
     assert lib.me_encode_tuning_get(n, 9, 256) == 0
     # the partials workspace covers every variant: 8-coil groups need more at 9 coils

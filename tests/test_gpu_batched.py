@@ -99,7 +99,7 @@ def test_batched_graph_matches_one_at_a_time(me):
         assert len(bj.encoder_tuning) == len({(sp["shape"], sp["coils"]) for sp in specs})
         for enc in bj.encoder_tuning.values():
             for variant, ms in enc.values():
-                assert variant in (0, 1, 2) and min(ms) > 0
+                assert variant in (0, 1, 2, 3) and min(ms) > 0
         ref = []
         for v in vols:
             jr = me.JointRecon(v["ksp"], v["mps"], v["shot_mask"], v["kgrid"], v["rkgrid"], max_joint_iter=n_iter,

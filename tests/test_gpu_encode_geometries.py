@@ -91,7 +91,7 @@ def test_normal_operator_and_rhs(case):
 
 def test_forward_variants_bitwise_equal_and_at_parity(me, case):
     """Every forward-encoding variant (16 coils x 2 samples per thread, 8 x 2,
-    8 x 4; me_encode_tuning_set) gives bitwise the same samples, residual
+    8 x 4, 4 x 4; me_encode_tuning_set) gives bitwise the same samples, residual
     norms equal to FP64 rounding, and the oracle's encoding."""
     d = case
     pb, shape, nc = d["pb"], d["shape"], d["nc"]
@@ -100,7 +100,7 @@ def test_forward_variants_bitwise_equal_and_at_parity(me, case):
     key = (L.int3(shape), nc, pb.plan.max_cnt)
     outs = []
     try:
-        for v in (0, 1, 2):
+        for v in (0, 1, 2, 3):
             L.call("me_encode_tuning_set", *key, v)
             assert L.size("me_encode_tuning_get", *key) == v
             Y = pb.forward(x).clone()
@@ -132,7 +132,7 @@ def test_forward_encoder_tuner_records_its_pick(me, case):
     try:
         picks = me.batched.tune_encoders(pb, reps=2)
         (variant, ms), = picks.values()
-        assert variant in (0, 1, 2) and min(ms) > 0
+        assert variant in (0, 1, 2, 3) and min(ms) > 0
         assert ms[variant] == min(ms)
         assert L.size("me_encode_tuning_get", L.int3(shape), nc, pb.plan.max_cnt) == variant
     finally:
