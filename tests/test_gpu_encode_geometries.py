@@ -26,6 +26,7 @@ CASES = [
     ((96, 24, 8), 3, (8, 4), 2, 4),    # N0 = 96 = 3 * 32, 279-281 samples: 2 forward chunks
     ((16, 384, 8), 4, (4, 4), 2, 4),   # N1 = 384: row-group adjoint, power-of-two N0
     ((12, 512, 8), 3, (4, 4), 2, 4),   # N1 = 512, N0 = 12: row-group adjoint with multiply-and-correct
+    ((32, 256, 8), 5, (4, 4), 1, 2),   # N1 = 256 (16-thread rows), 4096 samples: 8 forward chunks
 ]
 
 
@@ -56,7 +57,8 @@ def case(request, me):
 def test_case_selects_the_intended_paths(case):
     """The sample counts that make each case cover several chunks."""
     cnt = int(case["pb"].plan.cnt_h.max())
-    need = {(24, 12, 8): 129, (48, 32, 8): 513, (96, 24, 8): 257, (16, 384, 8): 513, (12, 512, 8): 513}
+    need = {(24, 12, 8): 129, (48, 32, 8): 513, (96, 24, 8): 257, (16, 384, 8): 513, (12, 512, 8): 513,
+            (32, 256, 8): 4096}
     assert cnt >= need[case["shape"]], cnt
 
 
