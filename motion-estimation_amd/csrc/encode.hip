@@ -214,17 +214,21 @@ __global__ void __launch_bounds__((FwdT<NR, G>::NT)) k_encode_fwd(EncArgs A) {
 #pragma unroll
       for (int k2 = 0; k2 < R2; ++k2) zrow[(t + R1 * k2) * ZS + g] = X[k2];
     }
+    // the samples' column twiddles do not depend on this row's spectra
+    float2 wj[SPT];
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) wj[j] = twc[tix[j]];
     __syncthreads();
+    // every thread accumulates all its samples (a slot past the shot's count
+    // has k0 = k1 = 0 and an accumulator that is never stored), so the
+    // samples share one block and their reads overlap
 #pragma unroll
     for (int j = 0; j < SPT; ++j) {
-      if (mv[j]) {
-        const float2 w = twc[tix[j]];
 #pragma unroll
-        for (int q = 0; q < G; q += 2) {
-          const float4 z = *reinterpret_cast<const float4*>(&zrow[mk1[j] * ZS + q]);
-          acc[j][q] = c_fma(w, make_float2(z.x, z.y), acc[j][q]);
-          acc[j][q + 1] = c_fma(w, make_float2(z.z, z.w), acc[j][q + 1]);
-        }
+      for (int q = 0; q < G; q += 2) {
+        const float4 z = *reinterpret_cast<const float4*>(&zrow[mk1[j] * ZS + q]);
+        acc[j][q] = c_fma(wj[j], make_float2(z.x, z.y), acc[j][q]);
+        acc[j][q + 1] = c_fma(wj[j], make_float2(z.z, z.w), acc[j][q + 1]);
       }
       tix[j] += mk0[j];
       tix[j] -= (tix[j] >= n0) ? n0 : 0;
