@@ -151,8 +151,30 @@ __global__ void __launch_bounds__((LineT<N, CONTIG, LL>::NT)) k_linepass(LineArg
 #pragma unroll
   for (int i = 0; i < (REDUCE ? R1 : 1); ++i) red[i] = make_float2(0.f, 0.f);
 
+  // The line's values of every input (MODE 0/1) are loaded before any of
+  // them is transformed, so the second input's loads are not held back by the
+  // barriers of the first one's transform.  (Every value is read before any
+  // is written: a workgroup owns its lines in every buffer.)  A REDUCE pass
+  // loads each batch entry at the top of its iteration: loading the next one
+  // during the current one's transform costs it occupancy (157 -> 222 VGPRs)
+  // and made it slower (1.53 -> 1.65 ms at 64 volumes of 256x256x192).
+  constexpr int NRAW = MODE == 2 ? 1 : R1;
+  float2 raw[NIN][NRAW];
+  auto load_raw = [&](int bb) {
+    if (MODE != 2 && t < R2) {
+      const long long step = (long long)R2 * A.st_a;
+#pragma unroll
+      for (int i = 0; i < NIN; ++i) {
+        const float2* st = A.in[i] + (long long)bb * A.in_bs[i] + line_off + (long long)t * A.st_a;
+#pragma unroll
+        for (int n1 = 0; n1 < NRAW; ++n1) raw[i][n1] = st[n1 * step];
+      }
+    }
+  };
+
   for (int b = b_begin; b < b_end; ++b) {
     const double* cf = A.coef ? A.coef + (long long)b * A.ncoef : nullptr;
+    load_raw(b);
     // ---------------- forward transforms of every input
     float2 X[NIN][R2];
 #pragma unroll
@@ -169,10 +191,8 @@ __global__ void __launch_bounds__((LineT<N, CONTIG, LL>::NT)) k_linepass(LineArg
       }
       if (t < R2) {
         float2 a[R1];
-        const float2* st = src + (long long)t * A.st_a;
-        const long long step = (long long)R2 * A.st_a;
 #pragma unroll
-        for (int n1 = 0; n1 < R1; ++n1) a[n1] = st[n1 * step];
+        for (int n1 = 0; n1 < R1; ++n1) a[n1] = raw[i][n1 < NRAW ? n1 : 0];
         RDFT<R1, -1>::run(a);
 #pragma unroll
         for (int k1 = 1; k1 < R1; ++k1) a[k1] = c_mul(a[k1], tw[(t * k1) % N]);
