@@ -101,7 +101,13 @@ __device__ __forceinline__ double block_sum_f64(double v, double* sh) {
   return s;
 }
 
-template <int NR, int SPT, int G>
+// FULL: the launch's largest shot fills whole chunks (max_cnt a multiple of
+// SPT * NT), so the sample slots are accumulated without a per-lane test (a
+// slot past its shot's count has k0 = k1 = 0 and an accumulator that is never
+// stored) and share one block, their spectrum reads in flight together;
+// otherwise only the lanes whose slot holds a sample accumulate.  Both give
+// bitwise the same samples.
+template <int NR, int SPT, int G, bool FULL>
 __global__ void __launch_bounds__((FwdT<NR, G>::NT)) k_encode_fwd(EncArgs A) {
   constexpr int R1 = FwdT<NR, G>::R1, R2 = FwdT<NR, G>::R2, T = FwdT<NR, G>::T, NT = FwdT<NR, G>::NT;
   // exchange rows padded to XS (even, so each row starts 16-byte aligned and
@@ -219,16 +225,15 @@ __global__ void __launch_bounds__((FwdT<NR, G>::NT)) k_encode_fwd(EncArgs A) {
 #pragma unroll
     for (int j = 0; j < SPT; ++j) wj[j] = twc[tix[j]];
     __syncthreads();
-    // every thread accumulates all its samples (a slot past the shot's count
-    // has k0 = k1 = 0 and an accumulator that is never stored), so the
-    // samples share one block and their reads overlap
 #pragma unroll
     for (int j = 0; j < SPT; ++j) {
+      if (FULL || mv[j]) {
 #pragma unroll
-      for (int q = 0; q < G; q += 2) {
-        const float4 z = *reinterpret_cast<const float4*>(&zrow[mk1[j] * ZS + q]);
-        acc[j][q] = c_fma(wj[j], make_float2(z.x, z.y), acc[j][q]);
-        acc[j][q + 1] = c_fma(wj[j], make_float2(z.z, z.w), acc[j][q + 1]);
+        for (int q = 0; q < G; q += 2) {
+          const float4 z = *reinterpret_cast<const float4*>(&zrow[mk1[j] * ZS + q]);
+          acc[j][q] = c_fma(wj[j], make_float2(z.x, z.y), acc[j][q]);
+          acc[j][q + 1] = c_fma(wj[j], make_float2(z.z, z.w), acc[j][q + 1]);
+        }
       }
       tix[j] += mk0[j];
       tix[j] -= (tix[j] >= n0) ? n0 : 0;
@@ -822,7 +827,10 @@ struct FwdLaunch {
   hipStream_t s;
   template <int NR>
   void operator()() {
-    hipLaunchKernelGGL((k_encode_fwd<NR, SPT, G>), grid, dim3(FwdT<NR, G>::NT), 0, s, *a);
+    if (a->max_cnt % (SPT * FwdT<NR, G>::NT) == 0)
+      hipLaunchKernelGGL((k_encode_fwd<NR, SPT, G, true>), grid, dim3(FwdT<NR, G>::NT), 0, s, *a);
+    else
+      hipLaunchKernelGGL((k_encode_fwd<NR, SPT, G, false>), grid, dim3(FwdT<NR, G>::NT), 0, s, *a);
   }
 };
 bool launch_fwd(int variant, int n1, const EncArgs* a, dim3 grid, hipStream_t s) {
@@ -964,6 +972,7 @@ int encode_forward_impl(const void* u, long long u_bstride, const void* maps, in
   A.ncg = fg.ncg;
   A.nchunk = fg.nchunk;
   A.blocks_per_shot = fg.blocks_per_shot;
+  A.max_cnt = max_cnt;
   A.nbatch = nbatch;
   ME_REQUIRE((long long)A.blocks_per_shot * nbatch < (1LL << 31), "encode_forward: grid too large");
   dim3 grid((unsigned)(A.blocks_per_shot * nbatch), 1u);
