@@ -274,13 +274,19 @@ __global__ void __launch_bounds__((LineT<N, CONTIG, LL>::NT)) k_linepass(LineArg
         } else if (valid) {
           float2* dt = dst + (long long)t * A.st_a;
           const long long step = (long long)R2 * A.st_a;
-          const unsigned char* mk = A.mask ? A.mask + (long long)b * A.mask_bs + line_off + (long long)t * A.st_a : nullptr;
+          if (!A.mask && !A.accumulate) {  // the common case: stores only, no per-element tests
 #pragma unroll
-          for (int n1 = 0; n1 < R1; ++n1) {
-            float2 val = MODE == 0 ? a[n1] : c_scale(a[n1], out_scale);
-            if (mk && !mk[n1 * step]) val = make_float2(0.f, 0.f);
-            if (A.accumulate) val = c_add(val, dt[n1 * step]);
-            dt[n1 * step] = val;
+            for (int n1 = 0; n1 < R1; ++n1) dt[n1 * step] = MODE == 0 ? a[n1] : c_scale(a[n1], out_scale);
+          } else {
+            const unsigned char* mk =
+                A.mask ? A.mask + (long long)b * A.mask_bs + line_off + (long long)t * A.st_a : nullptr;
+#pragma unroll
+            for (int n1 = 0; n1 < R1; ++n1) {
+              float2 val = MODE == 0 ? a[n1] : c_scale(a[n1], out_scale);
+              if (mk && !mk[n1 * step]) val = make_float2(0.f, 0.f);
+              if (A.accumulate) val = c_add(val, dt[n1 * step]);
+              dt[n1 * step] = val;
+            }
           }
         }
       }
