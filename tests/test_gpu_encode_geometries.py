@@ -35,9 +35,7 @@ def _rel(a, b):
     return float(np.linalg.norm(a - np.asarray(b)) / np.linalg.norm(b))
 
 
-@pytest.fixture(scope="module", params=CASES, ids=lambda c: "x".join(map(str, c[0])))
-def case(request, me):
-    shape, nc, tile, accel, nshots = request.param
+def _make_case(me, shape, nc, tile, accel, nshots):
     sim = me.simulation
     mps = sim.coil_maps(shape, nc, seed=21).numpy().astype(np.complex128)
     A, _ = sim.disorder_sampling(shape, tile, accel, nshots=nshots)
@@ -52,6 +50,11 @@ def case(request, me):
     pb.theta.copy_(torch.as_tensor(t))
     M = np.sum(np.abs(mps_u) ** 2, axis=0) > 0.1
     return dict(shape=shape, nc=nc, pb=pb, mps_u=mps_u, A_u=A_u, ksp=ksp, t=t, okg=okg, orkg=orkg, M=M, rng=rng)
+
+
+@pytest.fixture(scope="module", params=CASES, ids=lambda c: "x".join(map(str, c[0])))
+def case(request, me):
+    return _make_case(me, *request.param)
 
 
 def test_case_selects_the_intended_paths(case):
@@ -139,3 +142,31 @@ def test_forward_encoder_tuner_records_its_pick(me, case):
         assert L.size("me_encode_tuning_get", L.int3(shape), nc, pb.plan.max_cnt) == variant
     finally:
         L.load().me_encode_tuning_clear()
+
+
+def test_whole_and_partial_chunk_forms_bitwise_equal(me):
+    """The forward encoding accumulates its sample slots without a per-lane
+    test when max_cnt fills whole chunks and with one otherwise; the same
+    shots encoded under both forms (max_cnt as is, 4096, a multiple of every
+    variant's chunk, then one more, which adds an empty chunk) give bitwise
+    the same samples and residual norms."""
+    d = _make_case(me, (32, 256, 8), 5, (4, 4), 1, 2)
+    pb, shape, nc = d["pb"], d["shape"], d["nc"]
+    L = me._lib
+    x = pb.image_to_plane(d["rng"].standard_normal(shape) + 1j * d["rng"].standard_normal(shape))
+    U = torch.empty((pb.nshots, pb.vol), dtype=torch.complex64, device=pb.device)
+    pb._rigid(x, 0, U, pb.theta, 0)
+    pl = pb.plan
+    maxc = int(pl.cnt_h.max())
+    outs = []
+    for m in (maxc, maxc + 1):
+        Y = torch.zeros_like(pb.ymeas)
+        nps = int(L.size("me_encode_partials_size", shape[2], nc, m, shape[1]))
+        part = torch.empty(max(1, nps * pb.nshots), dtype=torch.float64, device=pb.device)
+        norms = torch.empty(pb.nshots, dtype=torch.float64, device=pb.device)
+        L.call("me_encode_forward", L.ptr(U), pb.vol, L.ptr(pb.maps), nc, pb.nshots, L.int3(shape), L.ptr(pl.sk0),
+               L.ptr(pl.sk1), L.ptr(pl.off), L.ptr(pl.cnt), m, L.ptr(Y), L.ptr(pb.ymeas), L.ptr(part),
+               L.ptr(norms), 1, L.stream(pb.device))
+        outs.append((Y, norms))
+    assert torch.equal(outs[0][0], outs[1][0])
+    assert float((outs[0][1] - outs[1][1]).abs().max() / outs[0][1].abs().max()) < 1e-12
