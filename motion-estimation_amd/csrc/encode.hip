@@ -86,6 +86,16 @@ struct EncArgs {
   int store_raw;  // forward: store Y itself even when meas is given (norms are still of Y - meas)
 };
 
+// Workgroups are dealt to the 8 XCDs round robin (workgroup i to XCD i % 8).
+// This maps them so that XCD x runs one contiguous range of the logical
+// order, so the shots of one readout plane (consecutive in the logical
+// order) share that XCD's L2 for the plane's map rows.
+__device__ __forceinline__ int xcd_block(int i, int n) {
+  const int per = n / 8, rem = n % 8;
+  const int x = i % 8, k = i / 8;
+  return x * per + min(x, rem) + k;
+}
+
 __device__ __forceinline__ double block_sum_f64(double v, double* sh) {
   // wave64 shuffle reduction, then one value per wave through LDS
 #pragma unroll
@@ -126,7 +136,8 @@ __global__ void __launch_bounds__((FwdT<NR, G>::NT)) k_encode_fwd(EncArgs A) {
   const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
   // block index: shot fastest, so the workgroups of all shots for one
   // (plane, coil group) run together and share the map rows through L2
-  int bx = blockIdx.x;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  int bx = lb;
   const int b = bx % A.nbatch;
   bx /= A.nbatch;
   const int x2 = bx % n2;
@@ -260,7 +271,7 @@ __global__ void __launch_bounds__((FwdT<NR, G>::NT)) k_encode_fwd(EncArgs A) {
   }
   if (A.partial) {
     const double s = block_sum_f64(local, red);
-    if (tid == 0) A.partial[(long long)b * A.blocks_per_shot + blockIdx.x / A.nbatch] = s;
+    if (tid == 0) A.partial[(long long)b * A.blocks_per_shot + lb / A.nbatch] = s;
   }
 }
 
@@ -328,9 +339,10 @@ __global__ void __launch_bounds__(AdjT<NR>::NT) k_encode_adj_rows(EncArgs A) {
   int* cp = (int*)(smem + AT::OFF_CP);
 
   const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
-  const int b = blockIdx.x % A.nbatch;  // shot fastest (map rows shared through L2)
-  const int x2 = (blockIdx.x / A.nbatch) % n2;
-  const int row0 = (blockIdx.x / A.nbatch / n2) * L;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int b = lb % A.nbatch;  // shot fastest (map rows shared through L2)
+  const int x2 = (lb / A.nbatch) % n2;
+  const int row0 = (lb / A.nbatch / n2) * L;
   const int tid = threadIdx.x;
   const int r = tid % L, t = tid / L;
   const int x0 = row0 + r;
@@ -538,9 +550,10 @@ __global__ void __launch_bounds__(AdjW<NR>::NT) k_encode_adj_wave(EncArgs A) {
   int* cp = (int*)(smem + AW::OFF_CP);
 
   const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
-  const int b = blockIdx.x % A.nbatch;  // shot fastest (map rows shared through L2)
-  const int x2 = (blockIdx.x / A.nbatch) % n2;
-  const int row0 = (blockIdx.x / A.nbatch / n2) * L;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int b = lb % A.nbatch;  // shot fastest (map rows shared through L2)
+  const int x2 = (lb / A.nbatch) % n2;
+  const int row0 = (lb / A.nbatch / n2) * L;
   const int tid = threadIdx.x;
   // scatter layout: lane = row, wavefront = column phase
   const int r = tid % L, t = tid / L;
