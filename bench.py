@@ -41,7 +41,8 @@ def parse():
                     help="skip the CPU-baseline leg: the oracle timing and the small-size FP64-oracle accuracy check")
     ap.add_argument("--no-objective", action="store_true", help="skip the per-iteration objective")
     ap.add_argument("--no-tune", action="store_true",
-                    help="skip the autotuners at setup (line passes and forward encoding; every level with --full)")
+                    help="skip the autotuners at setup (line passes and forward encoding; also with --grad, and at "
+                         "every level with --full)")
     ap.add_argument("--grad", action="store_true",
                     help="differentiation workload instead of the headline: data-consistency loss forward+backward "
                          "at c3 (192x192x128, 24 coils, 48 shots), recompute on/off, and a 64-direction central "
@@ -502,6 +503,10 @@ def grad_bench(args, torch, me, dev):
     shape, ncoils, nshots = cfg["shape"], cfg["coils"], cfg["nshots"]
     kg, rkg = me.compute_transform_grids_voxel(shape, (1, 1, 1))
     pb = me.encoding.SenseProblem(data["ksp"], data["mps"], data["A"], kg, rkg, centered=True)
+    tuning = None
+    if not args.no_tune:
+        tuning = dict(line_passes=me.batched.tune_problem(pb), encoders=me.batched.tune_encoders(pb))
+        log(f"grad: tuning {tuning}")
     x0 = torch.fft.ifftshift(data["gt"].to(dev), dim=(0, 1, 2)).to(torch.complex64)
     gen = torch.Generator(device=dev)
     gen.manual_seed(5)
@@ -548,7 +553,10 @@ def grad_bench(args, torch, me, dev):
         "value": round(1e3 / res["fwd_bwd_recompute"], 3), "unit": "fwd+bwd/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "higher_is_better": True, "dtype": "c64", "data": "synthetic",
         "config": {"workload": f"c3: {shape[0]}x{shape[1]}x{shape[2]} complex64, {ncoils} coils, {nshots} "
-                               f"DISORDER shots, gradients w.r.t. image and 48x6 motion"},
+                               f"DISORDER shots, gradients w.r.t. image and 48x6 motion",
+                   "tuning": None if tuning is None else {
+                       "line_passes": tuning["line_passes"][0],
+                       "encoders": {str(k): v[0] for k, v in tuning["encoders"].items()}}},
         "ms": {k: round(v, 3) for k, v in res.items()},
         "fd_check": {"directions": 64, "step": h, "median_rel": float(np.median(rel)), "max_rel": float(max(rel)),
                      "max_rel_to_gradient_scale": float(max(scaled))},
