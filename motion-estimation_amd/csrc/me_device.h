@@ -82,7 +82,7 @@ constexpr CS ce_cs_2pi(long j, long n) {
   long q = (4 * j) / n;
   long rem4 = 4 * j - q * n;  // 4*(j - q*n/4)
   double a = (kPi / 2.0) * (double)rem4 / (double)n;  // in [0, pi/2)
-  double c, s;
+  double c = 0.0, s = 0.0;
   if (a <= kPi / 4) {
     c = ce_cos_poly(a);
     s = ce_sin_poly(a);
