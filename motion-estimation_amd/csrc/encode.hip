@@ -236,6 +236,7 @@ __global__ void __launch_bounds__((FwdT<NR, G>::NT)) k_encode_fwd(EncArgs A) {
 #pragma unroll
     for (int j = 0; j < SPT; ++j) wj[j] = twc[tix[j]];
     __syncthreads();
+    __builtin_amdgcn_s_setprio(1);
 #pragma unroll
     for (int j = 0; j < SPT; ++j) {
       if (FULL || mv[j]) {
@@ -249,6 +250,7 @@ __global__ void __launch_bounds__((FwdT<NR, G>::NT)) k_encode_fwd(EncArgs A) {
       tix[j] += mk0[j];
       tix[j] -= (tix[j] >= n0) ? n0 : 0;
     }
+    __builtin_amdgcn_s_setprio(0);
   }
 
   const float scale = rsqrtf((float)n0 * (float)n1);
