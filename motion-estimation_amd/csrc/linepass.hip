@@ -32,6 +32,22 @@ namespace me {
 // (me_tune_line_passes times the variants on the caller's buffers).
 constexpr int kLinesDefault = 16;
 
+// A pass reads every line value once and writes it once: streaming
+// (nontemporal) loads and stores.  Together they took the 10-pass transform
+// of 64 volumes at 256x256x192 from 24.0 to 23.2 ms; either one alone changed
+// nothing.
+typedef float f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 stream_load(const float2* p) {
+  const f2v w = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+  return make_float2(w.x, w.y);
+}
+__device__ __forceinline__ void stream_store(float2* p, float2 v) {
+  f2v w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<f2v*>(p));
+}
+
 template <int N, bool CONTIG, int LL = kLinesDefault>
 struct LineT {
   static constexpr int R1 = LineFactor<N>::R1;
@@ -167,7 +183,7 @@ __global__ void __launch_bounds__((LineT<N, CONTIG, LL>::NT)) k_linepass(LineArg
       for (int i = 0; i < NIN; ++i) {
         const float2* st = A.in[i] + (long long)bb * A.in_bs[i] + line_off + (long long)t * A.st_a;
 #pragma unroll
-        for (int n1 = 0; n1 < NRAW; ++n1) raw[i][n1] = st[n1 * step];
+        for (int n1 = 0; n1 < NRAW; ++n1) raw[i][n1] = stream_load(st + n1 * step);
       }
     }
   };
@@ -276,7 +292,7 @@ __global__ void __launch_bounds__((LineT<N, CONTIG, LL>::NT)) k_linepass(LineArg
           const long long step = (long long)R2 * A.st_a;
           if (!A.mask && !A.accumulate) {  // the common case: stores only, no per-element tests
 #pragma unroll
-            for (int n1 = 0; n1 < R1; ++n1) dt[n1 * step] = MODE == 0 ? a[n1] : c_scale(a[n1], out_scale);
+            for (int n1 = 0; n1 < R1; ++n1) stream_store(dt + n1 * step, MODE == 0 ? a[n1] : c_scale(a[n1], out_scale));
           } else {
             const unsigned char* mk =
                 A.mask ? A.mask + (long long)b * A.mask_bs + line_off + (long long)t * A.st_a : nullptr;
