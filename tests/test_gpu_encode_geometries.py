@@ -7,7 +7,9 @@ kernel variants (image_estimation.py:94-128, transform_estimation.py:42-45):
   more than 16 threads);
 * shots with more samples per readout plane than one forward workgroup holds
   (several sample chunks per plane) and more than the adjoint stages at once
-  (512 per coil: several staged chunks).
+  (512 per coil: several staged chunks);
+* launch grids that are not a multiple of 8 workgroups (the workgroups are
+  mapped onto the 8 XCDs in contiguous ranges, the first ones one longer).
 
 Tolerance: relative L2 1e-4 on gathered k-space and images."""
 import numpy as np
@@ -27,6 +29,7 @@ CASES = [
     ((16, 384, 8), 4, (4, 4), 2, 4),   # N1 = 384: row-group adjoint, power-of-two N0
     ((12, 512, 8), 3, (4, 4), 2, 4),   # N1 = 512, N0 = 12: row-group adjoint with multiply-and-correct
     ((32, 256, 8), 5, (4, 4), 1, 2),   # N1 = 256 (16-thread rows), 4096 samples: 8 forward chunks
+    ((24, 12, 12), 3, (4, 4), 2, 3),   # 36-workgroup grids: not a multiple of the 8 XCDs
 ]
 
 
@@ -61,7 +64,7 @@ def test_case_selects_the_intended_paths(case):
     """The sample counts that make each case cover several chunks."""
     cnt = int(case["pb"].plan.cnt_h.max())
     need = {(24, 12, 8): 129, (48, 32, 8): 513, (96, 24, 8): 257, (16, 384, 8): 513, (12, 512, 8): 513,
-            (32, 256, 8): 4096}
+            (32, 256, 8): 4096, (24, 12, 12): 48}
     assert cnt >= need[case["shape"]], cnt
 
 
